@@ -1,0 +1,127 @@
+// gt_internal.hpp -- types shared by the HIP kernels and the host runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace gt {
+
+// Kept-mode rectangle of the half-spectrum in the compact device layout:
+// kx = 0..kx, ky = 0..p and -q..-1.  With the 2/3 rule on, p = q = M and
+// kx = M (M = largest index inside the truncation circle along an axis);
+// without it the rectangle is the whole half-spectrum (p = N/2, q = N/2-1).
+struct Rect {
+  int kx, p, q;
+  __host__ __device__ int ky() const { return p + 1 + q; }
+  __host__ __device__ int kc() const { return kx + 1; }
+};
+
+// Device-side scalars of commphy/commk (src/modules.f90:30-36,55-65).
+struct Phys {
+  int n;
+  double dkx, dky;
+  double ahalf;  // alpv/2 (nlt.f90:27)
+  int ahalf_is_one;
+  double dt, dt2, dt3, dt6;
+  double rtrunc2, fkxsmax, fkysmax;
+  int truncate;
+  double inv_n;  // 1/N (fft_dir.f90:19 divides by NY then NX)
+  double nu, mu, alpnu, alpmu, alpv;
+};
+
+// Arguments of the column (y) and row (x) passes of the nonlinear term.
+struct PassArgs {
+  Rect rc;
+  int kyp;     // pitch of a compact column (double2)
+  int ntiles;  // column tiles of the y-pass buffers
+  Phys ph;
+  const double2* tw;  // exp(-2 pi i m/N), m < N
+  double2* y1;        // 3 fields after the inverse y transform
+  double2* y2;        // 2 fields after the forward x transform
+  double2* z;         // stage state (compact)
+  double2* znew;      // RK4 accumulator (compact)
+  const double* emk;  // integrating factor (compact, real)
+  double2* out;       // nonlinear term (compact), NLT mode
+};
+
+enum CMode : int {
+  C_START = 0,  // inverse part only: stage input = z
+  C_NLT = 1,    // forward part only: out = N(q)
+  C_RK4_1 = 2,
+  C_RK4_2 = 3,
+  C_RK4_3 = 4,
+  C_RK4_4 = 5,
+  C_RK2_1 = 6,
+  C_RK2_2 = 7,
+};
+
+// Column-tile width of the y pass and row pairs per workgroup of the x pass,
+// per log2 N.  The y-pass LDS is BT lines of (N + N/16) double2.
+__host__ __device__ constexpr int bt_of(int logn) {
+  return logn <= 6 ? 64 : logn == 7 ? 32 : logn == 8 ? 16 : logn == 9 ? 8
+       : logn == 10 ? 4 : logn <= 12 ? 2 : 1;
+}
+__host__ __device__ constexpr int rp_of(int logn) {
+  return logn <= 6 ? (1 << (logn - 1)) : logn == 7 ? 32 : logn == 8 ? 16
+       : logn == 9 ? 8 : logn == 10 ? 4 : logn == 11 ? 2 : 1;
+}
+
+// Compact row r of FFT index p (ky in FFT order), or -1 outside the rect.
+__host__ __device__ inline int row_of(int p, const Rect& rc, int n) {
+  if (p <= rc.p) return p;
+  if (p >= n - rc.q) return p - (n - rc.ky());
+  return -1;
+}
+// FFT index of compact row r.
+__host__ __device__ inline int fft_of(int r, const Rect& rc, int n) {
+  return r <= rc.p ? r : n - rc.ky() + r;
+}
+// Signed ky index of FFT index p (src/defk.f90:25-30).
+__host__ __device__ inline int kyi_of(int p, int n) {
+  return p <= n / 2 ? p : p - n;
+}
+
+struct DiagArgs {
+  Rect rc;
+  int kyp;
+  Phys ph;
+  const double2* z;
+  const double2* znl;
+  double* part1;  // (kx+1) x NBIN partial sums
+  double* part2;
+  double* out1;   // NBIN
+  double* out2;
+  double* gpart;  // (kx+1) x 6
+  double* gout;   // 6
+};
+
+// Forced mode in the compact layout (src/iniforcing.f90 order).
+struct ForcedMode {
+  int idx;      // compact index of (ikxf, ikyf)
+  int pidx;     // compact index of the Hermitian partner, or -1
+  double ff1, ff2;
+};
+
+// Host-side launchers (gturbo_kernels.hip).
+hipError_t launch_c_pass(int logn, int mode, const PassArgs& a, hipStream_t s);
+hipError_t launch_r_pass(int logn, const PassArgs& a, hipStream_t s);
+hipError_t launch_ref_fft(int logn, int dir, double* z, const double2* tw,
+                          hipStream_t s);
+hipError_t launch_produ(int n, const double* z, double* u, double* v,
+                        hipStream_t s);
+hipError_t launch_trunc_ref(const Phys& ph, double* z, hipStream_t s);
+hipError_t launch_trunc_cmp(const Phys& ph, const Rect& rc, int kyp,
+                            double2* z, hipStream_t s);
+hipError_t launch_gather(const Phys& ph, const Rect& rc, int kyp,
+                         const double* ref, double2* cmp, int apply_trunc,
+                         int* outside_flag, hipStream_t s);
+hipError_t launch_scatter(const Phys& ph, const Rect& rc, int kyp,
+                          const double2* cmp, double* ref, hipStream_t s);
+hipError_t launch_forcing(const Phys& ph, int nf, const ForcedMode* modes,
+                          const double2* phases, double amp, double2* z,
+                          hipStream_t s);
+hipError_t launch_spectrum(const DiagArgs& d, int which, hipStream_t s);
+hipError_t launch_glob(const DiagArgs& d, hipStream_t s);
+hipError_t launch_shell_index(const Phys& ph, int* ib, hipStream_t s);
+
+}  // namespace gt

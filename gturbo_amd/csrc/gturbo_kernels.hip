@@ -1,0 +1,922 @@
+// gturbo_kernels.hip -- gfx950 kernels of the pseudospectral step.
+//
+// Nonlinear term N(q) = -ik.(q u)^ (src/nlt.f90) on the compact layout, as two
+// kinds of pass per evaluation:
+//
+//  * y pass (c_pass): one workgroup per tile of BT kx-columns.  Forward part:
+//    forward y transform of the two product fields, -ik. combine, 2/3
+//    truncation and the RK stage update (src/rkstep.f90 stage AXPYs).  Inverse
+//    part: from the next stage input, the spectral operator |k|^-alpha
+//    (src/nlt.f90:22-36) for q, u, v and their inverse y transforms.
+//  * x pass (r_pass): one workgroup per RP pairs of y rows.  Complex-to-real
+//    x transforms of q, u, v (two rows per complex transform), the products
+//    q u, q v (src/produ.f90), and the real-to-complex x transforms of the
+//    products (again two rows per transform).
+//
+// Between the passes the three (resp. two) fields live in HBM in a tiled
+// layout [field][tile][y][kx % BT]: the y pass reads and writes whole tiles
+// contiguously, the x pass reads and writes 2*BT*16-byte runs.
+#include "fft_core.hpp"
+#include "gt_internal.hpp"
+
+namespace gt {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ bool cut_mode(const Phys& ph, double fkxs,
+                                         double fkys) {
+  // src/trunc.f90:13-15
+  double k1 = fkys / ph.fkysmax;
+  double k2 = k1 + fkxs / ph.fkxsmax;
+  return k2 > ph.rtrunc2;
+}
+
+__device__ __forceinline__ double kalpha(const Phys& ph, double k2) {
+  // ka = k2**(alpv/2) (src/nlt.f90:27)
+  return ph.ahalf_is_one ? k2 : pow(k2, ph.ahalf);
+}
+
+template <int LOGN, int BT>
+__device__ __forceinline__ void load_tile(double2* lds,
+                                          const double2* __restrict__ src,
+                                          int tid) {
+  constexpr int N = 1 << LOGN, NTHR = BT * (N / 16), LINE = Line<LOGN>::PADDED;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    int e = tid + NTHR * i;
+    lds[(e % BT) * LINE + lpad(e / BT)] = src[e];
+  }
+}
+
+template <int LOGN, int BT>
+__device__ __forceinline__ void store_tile(double2* __restrict__ dst,
+                                           const double2* lds, int tid) {
+  constexpr int N = 1 << LOGN, NTHR = BT * (N / 16), LINE = Line<LOGN>::PADDED;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    int e = tid + NTHR * i;
+    dst[e] = lds[(e % BT) * LINE + lpad(e / BT)];
+  }
+}
+
+// Workgroup -> tile remap so that consecutive tiles run on one XCD
+// (workgroups are dealt round-robin over the 8 XCDs, each with its own L2).
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  if (nblk % 8) return bid;
+  return (bid % 8) * (nblk / 8) + bid / 8;
+}
+
+// ---------------------------------------------------------------------------
+// y pass
+// ---------------------------------------------------------------------------
+
+template <int LOGN, int MODE>
+__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
+    c_pass(PassArgs a) {
+  constexpr int N = 1 << LOGN, TPL = N / 16, BT = bt_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED;
+  constexpr bool FWD = MODE != C_START;
+  constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
+                       MODE == C_RK4_3 || MODE == C_RK2_1;
+  __shared__ double2 lds[BT * LINE];
+  const int tid = threadIdx.x, b = tid / TPL, j = tid % TPL;
+  const int tile = blockIdx.x;
+  const int c = tile * BT + b;
+  const Rect rc = a.rc;
+  const bool colok = c <= rc.kx;
+  double2* buf = lds + b * LINE;
+  const Phys& ph = a.ph;
+  const double fkx = (double)c * ph.dkx;
+  const double fkxs = fkx * fkx;
+  const size_t cbase = (size_t)c * a.kyp;
+  double2 s[16];
+
+  if constexpr (FWD) {
+    double2 A[16];
+    load_tile<LOGN, BT>(lds, a.y2 + (size_t)tile * N * BT, tid);
+    __syncthreads();
+    fft_line<LOGN, -1>(buf, j, a.tw);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) A[m] = buf[lpad(j + TPL * m)];
+    __syncthreads();
+    load_tile<LOGN, BT>(lds, a.y2 + (size_t)(a.ntiles + tile) * N * BT, tid);
+    __syncthreads();
+    fft_line<LOGN, -1>(buf, j, a.tw);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int p = j + TPL * m;
+      const int r = row_of(p, rc, N);
+      double2 Bv = buf[lpad(p)];
+      s[m] = make_double2(0.0, 0.0);
+      if (colok && r >= 0) {
+        const double fky = (double)kyi_of(p, N) * ph.dky;
+        const double fkys = fky * fky;
+        // fft_dir normalisation (src/fft_dir.f90:19), exact for N = 2^k
+        double2 Av = A[m];
+        Av.x = Av.x * ph.inv_n * ph.inv_n;
+        Av.y = Av.y * ph.inv_n * ph.inv_n;
+        Bv.x = Bv.x * ph.inv_n * ph.inv_n;
+        Bv.y = Bv.y * ph.inv_n * ph.inv_n;
+        // src/nlt.f90:57-58
+        double2 k = make_double2(fkx * Av.y + fky * Bv.y,
+                                 -fkx * Av.x - fky * Bv.x);
+        if (ph.truncate == 1 && cut_mode(ph, fkxs, fkys))
+          k = make_double2(0.0, 0.0);
+        const size_t q = cbase + r;
+        const bool mean = (c == 0 && p == 0);
+        if constexpr (MODE == C_NLT) {
+          if (mean) k = make_double2(0.0, 0.0);  // trunc.f90:24
+          a.out[q] = k;
+        } else if constexpr (MODE == C_RK4_1) {
+          // rkstep.f90:96,101
+          const double E = a.emk[q];
+          const double2 zz = a.z[q];
+          const double EE = E * E;
+          a.znew[q] = make_double2(EE * (zz.x + ph.dt6 * k.x),
+                                   EE * (zz.y + ph.dt6 * k.y));
+          s[m] = make_double2(E * (zz.x + ph.dt2 * k.x), E * (zz.y + ph.dt2 * k.y));
+        } else if constexpr (MODE == C_RK4_2) {
+          // rkstep.f90:105,110
+          const double E = a.emk[q];
+          const double2 zz = a.z[q];
+          const double2 zn = a.znew[q];
+          const double d3E = ph.dt3 * E;
+          a.znew[q] = make_double2(zn.x + d3E * k.x, zn.y + d3E * k.y);
+          s[m] = make_double2(E * zz.x + ph.dt2 * k.x, E * zz.y + ph.dt2 * k.y);
+        } else if constexpr (MODE == C_RK4_3) {
+          // rkstep.f90:114,119
+          const double E = a.emk[q];
+          const double2 zz = a.z[q];
+          const double2 zn = a.znew[q];
+          const double d3E = ph.dt3 * E;
+          a.znew[q] = make_double2(zn.x + d3E * k.x, zn.y + d3E * k.y);
+          s[m] = make_double2(E * (E * zz.x + ph.dt * k.x),
+                              E * (E * zz.y + ph.dt * k.y));
+        } else if constexpr (MODE == C_RK4_4) {
+          // rkstep.f90:123,127
+          const double2 zn = a.znew[q];
+          double2 zz = make_double2(zn.x + ph.dt6 * k.x, zn.y + ph.dt6 * k.y);
+          if (mean) zz = make_double2(0.0, 0.0);
+          a.z[q] = zz;
+        } else if constexpr (MODE == C_RK2_1) {
+          // rkstep.f90:24/34 and :42
+          const double E = a.emk[q];
+          const double2 zz = a.z[q];
+          s[m] = make_double2(E * (zz.x + ph.dt2 * k.x), E * (zz.y + ph.dt2 * k.y));
+          if (mean) s[m] = make_double2(0.0, 0.0);
+        } else if constexpr (MODE == C_RK2_2) {
+          // rkstep.f90:54/64 and :72
+          const double E = a.emk[q];
+          const double2 zz = a.z[q];
+          double2 zo = make_double2(E * (E * zz.x + ph.dt * k.x),
+                                    E * (E * zz.y + ph.dt * k.y));
+          if (mean) zo = make_double2(0.0, 0.0);
+          a.z[q] = zo;
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int p = j + TPL * m;
+      const int r = row_of(p, rc, N);
+      s[m] = (colok && r >= 0) ? a.z[cbase + r] : make_double2(0.0, 0.0);
+    }
+  }
+
+  if constexpr (INV) {
+    double ka[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int p = j + TPL * m;
+      const double fky = (double)kyi_of(p, N) * ph.dky;
+      const double k2 = fkxs + fky * fky;
+      const bool use = colok && row_of(p, rc, N) >= 0 && !(c == 0 && p == 0);
+      ka[m] = use ? kalpha(ph, k2) : 1.0;
+    }
+#pragma unroll 1
+    for (int f = 0; f < 3; ++f) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const int p = j + TPL * m;
+        const int r = row_of(p, rc, N);
+        double2 v = make_double2(0.0, 0.0);
+        if (colok && r >= 0) {
+          if (f == 0) {
+            v = s[m];
+          } else if (!(c == 0 && p == 0)) {  // nlt.f90:25
+            if (f == 1) {
+              const double kyn = ((double)kyi_of(p, N) * ph.dky) / ka[m];
+              v = make_double2(-kyn * s[m].y, kyn * s[m].x);  // nlt.f90:31-32
+            } else {
+              const double kxn = fkx / ka[m];
+              v = make_double2(kxn * s[m].y, -kxn * s[m].x);  // nlt.f90:33-34
+            }
+          }
+        }
+        buf[lpad(p)] = v;
+      }
+      __syncthreads();
+      fft_line<LOGN, +1>(buf, j, a.tw);
+      store_tile<LOGN, BT>(a.y1 + (size_t)(f * a.ntiles + tile) * N * BT, lds,
+                           tid);
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x pass
+// ---------------------------------------------------------------------------
+
+template <int LOGN>
+struct RowAddr {
+  static constexpr int N = 1 << LOGN, BT = bt_of(LOGN);
+  int ntiles;
+  __device__ __forceinline__ size_t operator()(int f, int k, int y) const {
+    return ((size_t)(f * ntiles + k / BT) * N + y) * BT + (k % BT);
+  }
+};
+
+// Z = X0 + i X1 over the full x-spectrum, from the half-spectra of two real
+// rows (kx <= kx_max; the imaginary parts at kx = 0 and N/2 are not used).
+template <int LOGN>
+__device__ __forceinline__ void build_pair(double2* buf, int j,
+                                           const double2* __restrict__ y,
+                                           const RowAddr<LOGN>& ad, int f,
+                                           int y0, int kxm) {
+  constexpr int N = 1 << LOGN, TPL = N / 16;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int p = j + TPL * m;
+    double2 v = make_double2(0.0, 0.0);
+    if (p == 0 || (p == N / 2 && kxm == N / 2)) {
+      const double2 x0 = y[ad(f, p, y0)], x1 = y[ad(f, p, y0 + 1)];
+      v = make_double2(x0.x, x1.x);
+    } else if (p < N / 2) {
+      if (p <= kxm) {
+        const double2 x0 = y[ad(f, p, y0)], x1 = y[ad(f, p, y0 + 1)];
+        v = make_double2(x0.x - x1.y, x0.y + x1.x);
+      }
+    } else if (p > N / 2) {
+      const int k = N - p;
+      if (k <= kxm) {
+        const double2 x0 = y[ad(f, k, y0)], x1 = y[ad(f, k, y0 + 1)];
+        v = make_double2(x0.x + x1.y, x1.x - x0.y);
+      }
+    }
+    buf[lpad(p)] = v;
+  }
+}
+
+// Split W = FFT(a0 + i a1) into the half-spectra of the real rows a0, a1 and
+// store kx <= kx_max.
+template <int LOGN>
+__device__ __forceinline__ void split_store(const double2* buf, int j,
+                                            double2* __restrict__ y,
+                                            const RowAddr<LOGN>& ad, int f,
+                                            int y0, int kxm) {
+  constexpr int N = 1 << LOGN, TPL = N / 16;
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const int k = j + TPL * m;
+    if (k <= kxm && (m < 8 || j == 0)) {
+      const double2 W = buf[lpad(k)];
+      const double2 Wc = buf[lpad((N - k) & (N - 1))];
+      y[ad(f, k, y0)] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
+      y[ad(f, k, y0 + 1)] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
+    }
+  }
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
+    r_pass(PassArgs a) {
+  constexpr int N = 1 << LOGN, TPL = N / 16, RP = rp_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED;
+  __shared__ double2 lds[RP * LINE];
+  const int pl = threadIdx.x / TPL, j = threadIdx.x % TPL;
+  const int g = xcd_remap(blockIdx.x, gridDim.x);
+  const int y0 = 2 * (g * RP + pl);
+  double2* buf = lds + pl * LINE;
+  const int kxm = a.rc.kx;
+  const RowAddr<LOGN> ad{a.ntiles};
+
+  double q0[16], q1[16], a0[16], a1[16];
+  build_pair<LOGN>(buf, j, a.y1, ad, 0, y0, kxm);
+  __syncthreads();
+  fft_line<LOGN, +1>(buf, j, a.tw);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const double2 v = buf[lpad(j + TPL * m)];
+    q0[m] = v.x;
+    q1[m] = v.y;
+  }
+  __syncthreads();
+  build_pair<LOGN>(buf, j, a.y1, ad, 1, y0, kxm);
+  __syncthreads();
+  fft_line<LOGN, +1>(buf, j, a.tw);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const double2 v = buf[lpad(j + TPL * m)];
+    a0[m] = q0[m] * v.x;  // produ.f90:15
+    a1[m] = q1[m] * v.y;
+  }
+  __syncthreads();
+  build_pair<LOGN>(buf, j, a.y1, ad, 2, y0, kxm);
+  __syncthreads();
+  fft_line<LOGN, +1>(buf, j, a.tw);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int x = lpad(j + TPL * m);
+    const double2 v = buf[x];
+    buf[x] = make_double2(q0[m] * v.x, q1[m] * v.y);  // produ.f90:16
+  }
+  __syncthreads();
+  fft_line<LOGN, -1>(buf, j, a.tw);
+  split_store<LOGN>(buf, j, a.y2, ad, 1, y0, kxm);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < 16; ++m) buf[lpad(j + TPL * m)] = make_double2(a0[m], a1[m]);
+  __syncthreads();
+  fft_line<LOGN, -1>(buf, j, a.tw);
+  split_store<LOGN>(buf, j, a.y2, ad, 0, y0, kxm);
+}
+
+// ---------------------------------------------------------------------------
+// Reference-layout 2D transforms (src/fft_dir.f90, src/fft_inv.f90):
+// real rows of N+2 doubles, half-spectrum rows of N/2+1 complex.
+// ---------------------------------------------------------------------------
+
+template <int LOGN>
+__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
+    ref_rows_fwd(double* z, const double2* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, TPL = N / 16, RP = rp_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED;
+  __shared__ double2 lds[RP * LINE];
+  const int pl = threadIdx.x / TPL, j = threadIdx.x % TPL;
+  const int y0 = 2 * (blockIdx.x * RP + pl);
+  double2* buf = lds + pl * LINE;
+  const size_t ld = (size_t)N + 2;
+  const double* r0 = z + ld * y0;
+  const double* r1 = r0 + ld;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int x = j + TPL * m;
+    buf[lpad(x)] = make_double2(r0[x], r1[x]);
+  }
+  __syncthreads();
+  fft_line<LOGN, -1>(buf, j, tw);
+  double2* c0 = reinterpret_cast<double2*>(z + ld * y0);
+  double2* c1 = c0 + (N / 2 + 1);
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const int k = j + TPL * m;
+    if (k <= N / 2 && (m < 8 || j == 0)) {
+      const double2 W = buf[lpad(k)];
+      const double2 Wc = buf[lpad((N - k) & (N - 1))];
+      c0[k] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
+      c1[k] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
+    }
+  }
+}
+
+template <int LOGN>
+__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
+    ref_rows_inv(double* z, const double2* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, TPL = N / 16, RP = rp_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED;
+  __shared__ double2 lds[RP * LINE];
+  const int pl = threadIdx.x / TPL, j = threadIdx.x % TPL;
+  const int y0 = 2 * (blockIdx.x * RP + pl);
+  double2* buf = lds + pl * LINE;
+  const size_t ld = (size_t)N + 2;
+  const double2* c0 = reinterpret_cast<const double2*>(z + ld * y0);
+  const double2* c1 = c0 + (N / 2 + 1);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int p = j + TPL * m;
+    double2 v;
+    if (p == 0 || p == N / 2) {
+      v = make_double2(c0[p].x, c1[p].x);
+    } else if (p < N / 2) {
+      const double2 x0 = c0[p], x1 = c1[p];
+      v = make_double2(x0.x - x1.y, x0.y + x1.x);
+    } else {
+      const double2 x0 = c0[N - p], x1 = c1[N - p];
+      v = make_double2(x0.x + x1.y, x1.x - x0.y);
+    }
+    buf[lpad(p)] = v;
+  }
+  __syncthreads();
+  fft_line<LOGN, +1>(buf, j, tw);
+  double* r0 = z + ld * y0;
+  double* r1 = r0 + ld;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int x = j + TPL * m;
+    const double2 v = buf[lpad(x)];
+    r0[x] = v.x;
+    r1[x] = v.y;
+  }
+  if (j < 2) {
+    r0[N + j] = 0.0;
+    r1[N + j] = 0.0;
+  }
+}
+
+// y transform of every kx column in place; SCALE applies /NY/NX.
+template <int LOGN, int S, int SCALE>
+__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
+    ref_cols(double* z, const double2* __restrict__ tw) {
+  constexpr int N = 1 << LOGN, TPL = N / 16, BT = bt_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED, NTHR = BT * TPL, NH = N / 2 + 1;
+  __shared__ double2 lds[BT * LINE];
+  const int tid = threadIdx.x, b = tid / TPL, j = tid % TPL;
+  const int kx0 = blockIdx.x * BT;
+  double2* zc = reinterpret_cast<double2*>(z);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int e = tid + NTHR * i, y = e / BT, bb = e % BT, kx = kx0 + bb;
+    lds[bb * LINE + lpad(y)] =
+        kx < NH ? zc[kx + (size_t)NH * y] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_line<LOGN, S>(lds + b * LINE, j, tw);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int e = tid + NTHR * i, y = e / BT, bb = e % BT, kx = kx0 + bb;
+    if (kx < NH) {
+      double2 v = lds[bb * LINE + lpad(y)];
+      if (SCALE) {
+        v.x = v.x / (double)N / (double)N;
+        v.y = v.y / (double)N / (double)N;
+      }
+      zc[kx + (size_t)NH * y] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers.  Each grid size is instantiated in its own translation unit
+// (compiled with -DGT_INST_LOGN=L); the dispatchers and the size-independent
+// kernels are in the one compiled with -DGT_INST_COMMON.
+// ---------------------------------------------------------------------------
+
+template <int L>
+hipError_t c_pass_launch(int mode, const PassArgs& a, hipStream_t s);
+template <int L>
+hipError_t r_pass_launch(const PassArgs& a, hipStream_t s);
+template <int L>
+hipError_t ref_fft_launch(int dir, double* z, const double2* tw, hipStream_t s);
+
+#if defined(GT_INST_LOGN)
+template <int L>
+hipError_t c_pass_launch(int mode, const PassArgs& a, hipStream_t s) {
+  constexpr int T = bt_of(L) * (1 << L) / 16;
+  dim3 g(a.ntiles), b(T);
+  switch (mode) {
+    case C_START: c_pass<L, C_START><<<g, b, 0, s>>>(a); break;
+    case C_NLT: c_pass<L, C_NLT><<<g, b, 0, s>>>(a); break;
+    case C_RK4_1: c_pass<L, C_RK4_1><<<g, b, 0, s>>>(a); break;
+    case C_RK4_2: c_pass<L, C_RK4_2><<<g, b, 0, s>>>(a); break;
+    case C_RK4_3: c_pass<L, C_RK4_3><<<g, b, 0, s>>>(a); break;
+    case C_RK4_4: c_pass<L, C_RK4_4><<<g, b, 0, s>>>(a); break;
+    case C_RK2_1: c_pass<L, C_RK2_1><<<g, b, 0, s>>>(a); break;
+    case C_RK2_2: c_pass<L, C_RK2_2><<<g, b, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int L>
+hipError_t r_pass_launch(const PassArgs& a, hipStream_t s) {
+  constexpr int RP = rp_of(L), T = RP * (1 << L) / 16;
+  const int npairs = (1 << L) / 2;
+  r_pass<L><<<dim3(npairs / RP), dim3(T), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <int L>
+hipError_t ref_fft_launch(int dir, double* z, const double2* tw, hipStream_t s) {
+  constexpr int N = 1 << L, RP = rp_of(L), BT = bt_of(L);
+  const dim3 gr(N / 2 / RP), br(RP * N / 16);
+  const dim3 gc((N / 2 + 1 + BT - 1) / BT), bc(BT * N / 16);
+  if (dir < 0) {  // fft_dir: rows R2C, columns forward, /NY/NX
+    ref_rows_fwd<L><<<gr, br, 0, s>>>(z, tw);
+    ref_cols<L, -1, 1><<<gc, bc, 0, s>>>(z, tw);
+  } else {  // fft_inv: columns inverse, rows C2R
+    ref_cols<L, +1, 0><<<gc, bc, 0, s>>>(z, tw);
+    ref_rows_inv<L><<<gr, br, 0, s>>>(z, tw);
+  }
+  return hipGetLastError();
+}
+
+template hipError_t c_pass_launch<GT_INST_LOGN>(int, const PassArgs&, hipStream_t);
+template hipError_t r_pass_launch<GT_INST_LOGN>(const PassArgs&, hipStream_t);
+template hipError_t ref_fft_launch<GT_INST_LOGN>(int, double*, const double2*, hipStream_t);
+#endif  // GT_INST_LOGN
+
+#if defined(GT_INST_COMMON)
+// ---------------------------------------------------------------------------
+// Elementwise operators
+// ---------------------------------------------------------------------------
+
+// src/produ.f90:11-23
+__global__ void produ_k(int n, const double* __restrict__ z, double* u,
+                        double* v) {
+  const size_t ld = (size_t)n + 2, tot = ld * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = e % ld;
+    if (i < (size_t)n) {
+      u[e] = z[e] * u[e];
+      v[e] = z[e] * v[e];
+    } else {
+      u[e] = 0.0;
+      v[e] = 0.0;
+    }
+  }
+}
+
+// src/trunc.f90:10-25 on the reference layout.
+__global__ void trunc_ref_k(Phys ph, double2* z) {
+  const int n = ph.n, nh = n / 2 + 1;
+  const size_t tot = (size_t)nh * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % nh), jj = (int)(e / nh);
+    const double fkx = (double)i * ph.dkx, fky = (double)kyi_of(jj, n) * ph.dky;
+    if (cut_mode(ph, fkx * fkx, fky * fky) || e == 0) z[e] = make_double2(0.0, 0.0);
+  }
+}
+
+__global__ void trunc_cmp_k(Phys ph, Rect rc, int kyp, double2* z) {
+  const int n = ph.n, ky = rc.ky();
+  const size_t tot = (size_t)rc.kc() * ky;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / ky), r = (int)(e % ky);
+    const int p = fft_of(r, rc, n);
+    const double fkx = (double)c * ph.dkx, fky = (double)kyi_of(p, n) * ph.dky;
+    if (cut_mode(ph, fkx * fkx, fky * fky) || (c == 0 && r == 0))
+      z[(size_t)c * kyp + r] = make_double2(0.0, 0.0);
+  }
+}
+
+// Reference half-spectrum -> compact rectangle (optionally truncated).
+__global__ void gather_k(Phys ph, Rect rc, int kyp, const double2* __restrict__ ref,
+                         double2* cmp, int apply_trunc) {
+  const int n = ph.n, nh = n / 2 + 1, ky = rc.ky(), kc = rc.kc();
+  const size_t tot = (size_t)kc * ky;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e % kc), r = (int)(e / kc);
+    const int p = fft_of(r, rc, n);
+    double2 v = ref[c + (size_t)nh * p];
+    if (apply_trunc) {
+      const double fkx = (double)c * ph.dkx, fky = (double)kyi_of(p, n) * ph.dky;
+      if (cut_mode(ph, fkx * fkx, fky * fky) || (c == 0 && p == 0))
+        v = make_double2(0.0, 0.0);
+    }
+    cmp[(size_t)c * kyp + r] = v;
+  }
+}
+
+// Flags a nonzero reference mode outside the rectangle.
+__global__ void outside_k(Phys ph, Rect rc, const double2* __restrict__ ref,
+                          int* flag) {
+  const int n = ph.n, nh = n / 2 + 1;
+  const size_t tot = (size_t)nh * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % nh), p = (int)(e / nh);
+    if (i > rc.kx || row_of(p, rc, n) < 0) {
+      const double2 v = ref[e];
+      if (v.x != 0.0 || v.y != 0.0) atomicOr(flag, 1);
+    }
+  }
+}
+
+// Compact rectangle -> reference half-spectrum (zeros outside).
+__global__ void scatter_k(Phys ph, Rect rc, int kyp, const double2* __restrict__ cmp,
+                          double2* ref) {
+  const int n = ph.n, nh = n / 2 + 1;
+  const size_t tot = (size_t)nh * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % nh), p = (int)(e / nh);
+    const int r = row_of(p, rc, n);
+    ref[e] = (i <= rc.kx && r >= 0) ? cmp[(size_t)i * kyp + r]
+                                    : make_double2(0.0, 0.0);
+  }
+}
+
+// src/forcing.f90:16-33.  One thread per forced mode: no two modes share an
+// element, and a mode's own updates (itself, then its Hermitian partner) run
+// in the reference's order.  amp = sdt*famp.
+__global__ void forcing_k(int nf, const ForcedMode* __restrict__ fm,
+                          const double2* __restrict__ ph, double amp,
+                          double2* z) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  const ForcedMode f = fm[k];
+  const double2 cs = ph[k];
+  const double a1 = amp * f.ff1 * cs.x, a2 = amp * f.ff2 * cs.y;
+  double2 v = z[f.idx];
+  v.x = v.x + a1;
+  v.y = v.y + a2;
+  z[f.idx] = v;
+  if (f.pidx >= 0) {
+    double2 w = z[f.pidx];
+    w.x = w.x + a1;
+    w.y = w.y - a2;
+    z[f.pidx] = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (src/diagnostics.f90), deterministic: per column the modes of a
+// shell form one run of rows in each half of the column (the shell index is
+// monotone in |ky|); runs are summed in row order into part[c][b], and the
+// columns are then summed in order of c.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int shell_of(const Phys& ph, double fkxs, int p) {
+  const double fky = (double)kyi_of(p, ph.n) * ph.dky;
+  const double k = sqrt(fkxs + fky * fky);
+  return (int)(k / ph.dkx) + 1;  // diagnostics.f90:32-34 (dk = dkx)
+}
+
+// first row r in [lo, hi) with pred(r) true, pred monotone false->true
+template <class P>
+__device__ __forceinline__ int lower_row(int lo, int hi, P pred) {
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (pred(mid)) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// which = 0: spectrum (sp, sz);  which = 1: fluxes (flzm, flem)
+__global__ void __launch_bounds__(256) shell_part_k(DiagArgs d, int which) {
+  const Phys& ph = d.ph;
+  const Rect rc = d.rc;
+  const int n = ph.n, nbin = n / 2, c = blockIdx.x;
+  const double fkx = (double)c * ph.dkx, fkxs = fkx * fkx;
+  const int P = rc.p, Ky = rc.ky();
+  auto ibr = [&](int r) { return shell_of(ph, fkxs, fft_of(r, rc, n)); };
+  const int blo = ibr(0);
+  int bhi = ibr(P);
+  if (Ky > P + 1) bhi = max(bhi, ibr(P + 1));
+  bhi = min(bhi, nbin - 1);
+  const double edge = (c == 0 || c == n / 2) ? 1.0 : 0.0;
+  const size_t cq = (size_t)c * d.kyp;
+  for (int b = blo + (int)threadIdx.x; b <= bhi; b += blockDim.x) {
+    double acc1 = 0.0, acc2 = 0.0;
+    // positive half: rows 0..P, shell non-decreasing in r
+    const int p0 = lower_row(0, P + 1, [&](int r) { return ibr(r) >= b; });
+    const int p1 = lower_row(p0, P + 1, [&](int r) { return ibr(r) > b; });
+    // negative half: rows P+1..Ky-1, shell non-increasing in r
+    const int n0 = lower_row(P + 1, Ky, [&](int r) { return ibr(r) <= b; });
+    const int n1 = lower_row(n0, Ky, [&](int r) { return ibr(r) < b; });
+    for (int seg = 0; seg < 2; ++seg) {
+      const int r0 = seg ? n0 : p0, r1 = seg ? n1 : p1;
+      for (int r = r0; r < r1; ++r) {
+        if (c == 0 && r == 0) continue;  // diagnostics.f90:28
+        const int p = fft_of(r, rc, n);
+        const double fky = (double)kyi_of(p, n) * ph.dky;
+        const double k2 = fkxs + fky * fky;
+        const double ka = kalpha(ph, k2);
+        const double2 zv = d.z[cq + r];
+        if (which == 0) {
+          const double fac = edge != 0.0 ? 0.5 : 1.0;
+          const double am = hypot(zv.x, zv.y);
+          const double a2 = am * am;
+          acc1 = acc1 + fac * a2 / ka;  // diagnostics.f90:36-37
+          acc2 = acc2 + fac * a2;
+        } else {
+          const double fac = edge != 0.0 ? 1.0 : 2.0;
+          const double2 nv = d.znl[cq + r];
+          const double fz = zv.x * nv.x + zv.y * nv.y;  // diagnostics.f90:93
+          const double fe = fz / ka;
+          acc1 = acc1 + fac * fz;  // flzm, diagnostics.f90:98
+          acc2 = acc2 + fac * fe;  // flem
+        }
+      }
+    }
+    d.part1[(size_t)c * nbin + (b - 1)] = acc1;
+    d.part2[(size_t)c * nbin + (b - 1)] = acc2;
+  }
+}
+
+__global__ void shell_reduce_k(DiagArgs d, int which) {
+  const Phys& ph = d.ph;
+  const Rect rc = d.rc;
+  const int n = ph.n, nbin = n / 2;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x + 1;  // shell 1..NBIN
+  if (b > nbin) return;
+  double s1 = 0.0, s2 = 0.0;
+  const int P = rc.p, Ky = rc.ky();
+  for (int c = 0; c <= rc.kx; ++c) {
+    const double fkx = (double)c * ph.dkx, fkxs = fkx * fkx;
+    const int blo = shell_of(ph, fkxs, 0);
+    if (blo > b) break;  // shells only grow with c
+    int bhi = shell_of(ph, fkxs, fft_of(P, rc, n));
+    if (Ky > P + 1) bhi = max(bhi, shell_of(ph, fkxs, fft_of(P + 1, rc, n)));
+    bhi = min(bhi, nbin - 1);
+    if (b <= bhi) {
+      s1 = s1 + d.part1[(size_t)c * nbin + (b - 1)];
+      s2 = s2 + d.part2[(size_t)c * nbin + (b - 1)];
+    }
+  }
+  if (which == 0) {  // diagnostics.f90:42-45
+    d.out1[b - 1] = s1 / ph.dkx;
+    d.out2[b - 1] = s2 / ph.dkx;
+  } else {
+    d.part1[(size_t)(rc.kx + 1) * nbin + (b - 1)] = s1;  // flzm
+    d.part2[(size_t)(rc.kx + 1) * nbin + (b - 1)] = s2;  // flem
+  }
+}
+
+// flz(i) = sum(flzm(i:NBIN)) in the reference's order (diagnostics.f90:104-109)
+__global__ void suffix_k(DiagArgs d) {
+  const int nbin = d.ph.n / 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbin) return;
+  const double* flzm = d.part1 + (size_t)(d.rc.kx + 1) * nbin;
+  const double* flem = d.part2 + (size_t)(d.rc.kx + 1) * nbin;
+  double sz = 0.0, se = 0.0;
+  for (int q = i; q < nbin; ++q) {
+    sz = sz + flzm[q];
+    se = se + flem[q];
+  }
+  d.out1[i] = se;  // fle
+  d.out2[i] = sz;  // flz
+}
+
+// src/diagnostics.f90:147-163, per column, then over columns in order.
+__global__ void __launch_bounds__(256) glob_part_k(DiagArgs d) {
+  const Phys& ph = d.ph;
+  const Rect rc = d.rc;
+  const int n = ph.n, c = blockIdx.x, Ky = rc.ky();
+  const double fkx = (double)c * ph.dkx, fkxs = fkx * fkx;
+  const double fac = (c == 0 || c == n / 2) ? 0.5 : 1.0;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int r = threadIdx.x; r < Ky; r += blockDim.x) {
+    if (c == 0 && r == 0) continue;
+    const int p = fft_of(r, rc, n);
+    const double fky = (double)kyi_of(p, n) * ph.dky;
+    const double k2 = fkxs + fky * fky;
+    const double ka = kalpha(ph, k2);
+    const double2 zv = d.z[(size_t)c * d.kyp + r];
+    const double am = hypot(zv.x, zv.y);
+    const double scra = am * am;
+    acc[0] = acc[0] + fac * scra / ka;
+    acc[1] = acc[1] + fac * scra;
+    acc[2] = acc[2] + 2.0 * fac * ph.nu * pow(k2, ph.alpnu) * scra;
+    acc[3] = acc[3] + 2.0 * fac * ph.mu * pow(k2, ph.alpmu) * scra;
+    acc[4] = acc[4] + 2.0 * fac * ph.nu * pow(k2, ph.alpnu - ph.alpv / 2.0) * scra;
+    acc[5] = acc[5] + 2.0 * fac * ph.mu * pow(k2, ph.alpmu - ph.alpv / 2.0) * scra;
+  }
+  __shared__ double red[6][256];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) d.gpart[(size_t)c * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void glob_reduce_k(DiagArgs d) {
+  const int q = threadIdx.x;
+  if (q >= 6) return;
+  double s = 0.0;
+  for (int c = 0; c <= d.rc.kx; ++c) s = s + d.gpart[(size_t)c * 6 + q];
+  d.gout[q] = s;
+}
+
+__global__ void shell_index_k(Phys ph, int* ib) {
+  const int n = ph.n, nh = n / 2 + 1;
+  const size_t tot = (size_t)nh * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % nh), p = (int)(e / nh);
+    const double fkx = (double)i * ph.dkx;
+    ib[e] = (e == 0) ? 0 : shell_of(ph, fkx * fkx, p);
+  }
+}
+
+static inline unsigned grid_for(size_t tot, int block) {
+  size_t g = (tot + block - 1) / block;
+  return (unsigned)(g > 65536 ? 65536 : (g == 0 ? 1 : g));
+}
+
+#define GT_LOGN_CASE(V, ...) \
+  case V: {                     \
+    constexpr int L = V;        \
+    __VA_ARGS__;                \
+  } break;
+#define GT_LOGN_SWITCH(LOGN_VAR, ...)                                     \
+  switch (LOGN_VAR) {                                                     \
+    GT_LOGN_CASE(4, __VA_ARGS__) GT_LOGN_CASE(5, __VA_ARGS__)             \
+    GT_LOGN_CASE(6, __VA_ARGS__) GT_LOGN_CASE(7, __VA_ARGS__)             \
+    GT_LOGN_CASE(8, __VA_ARGS__) GT_LOGN_CASE(9, __VA_ARGS__)             \
+    GT_LOGN_CASE(10, __VA_ARGS__) GT_LOGN_CASE(11, __VA_ARGS__)           \
+    GT_LOGN_CASE(12, __VA_ARGS__) GT_LOGN_CASE(13, __VA_ARGS__)           \
+    default: return hipErrorInvalidValue;                                 \
+  }
+
+hipError_t launch_c_pass(int logn, int mode, const PassArgs& a, hipStream_t s) {
+  GT_LOGN_SWITCH(logn, return c_pass_launch<L>(mode, a, s));
+  return hipSuccess;
+}
+
+hipError_t launch_r_pass(int logn, const PassArgs& a, hipStream_t s) {
+  GT_LOGN_SWITCH(logn, return r_pass_launch<L>(a, s));
+  return hipSuccess;
+}
+
+hipError_t launch_ref_fft(int logn, int dir, double* z, const double2* tw,
+                          hipStream_t s) {
+  GT_LOGN_SWITCH(logn, return ref_fft_launch<L>(dir, z, tw, s));
+  return hipSuccess;
+}
+
+hipError_t launch_produ(int n, const double* z, double* u, double* v,
+                        hipStream_t s) {
+  produ_k<<<grid_for(((size_t)n + 2) * n, 256), 256, 0, s>>>(n, z, u, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_trunc_ref(const Phys& ph, double* z, hipStream_t s) {
+  trunc_ref_k<<<grid_for((size_t)(ph.n / 2 + 1) * ph.n, 256), 256, 0, s>>>(
+      ph, reinterpret_cast<double2*>(z));
+  return hipGetLastError();
+}
+
+hipError_t launch_trunc_cmp(const Phys& ph, const Rect& rc, int kyp, double2* z,
+                            hipStream_t s) {
+  trunc_cmp_k<<<grid_for((size_t)rc.kc() * rc.ky(), 256), 256, 0, s>>>(ph, rc, kyp, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const Phys& ph, const Rect& rc, int kyp, const double* ref,
+                         double2* cmp, int apply_trunc, int* outside_flag,
+                         hipStream_t s) {
+  const double2* rc2 = reinterpret_cast<const double2*>(ref);
+  gather_k<<<grid_for((size_t)rc.kc() * rc.ky(), 256), 256, 0, s>>>(
+      ph, rc, kyp, rc2, cmp, apply_trunc);
+  if (outside_flag)
+    outside_k<<<grid_for((size_t)(ph.n / 2 + 1) * ph.n, 256), 256, 0, s>>>(
+        ph, rc, rc2, outside_flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Phys& ph, const Rect& rc, int kyp, const double2* cmp,
+                          double* ref, hipStream_t s) {
+  scatter_k<<<grid_for((size_t)(ph.n / 2 + 1) * ph.n, 256), 256, 0, s>>>(
+      ph, rc, kyp, cmp, reinterpret_cast<double2*>(ref));
+  return hipGetLastError();
+}
+
+hipError_t launch_forcing(const Phys& ph, int nf, const ForcedMode* modes,
+                          const double2* phases, double amp, double2* z,
+                          hipStream_t s) {
+  (void)ph;
+  if (nf <= 0) return hipSuccess;
+  forcing_k<<<(nf + 255) / 256, 256, 0, s>>>(nf, modes, phases, amp, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_spectrum(const DiagArgs& d, int which, hipStream_t s) {
+  const int nbin = d.ph.n / 2;
+  shell_part_k<<<d.rc.kc(), 256, 0, s>>>(d, which);
+  shell_reduce_k<<<(nbin + 255) / 256, 256, 0, s>>>(d, which);
+  if (which == 1) suffix_k<<<(nbin + 255) / 256, 256, 0, s>>>(d);
+  return hipGetLastError();
+}
+
+hipError_t launch_glob(const DiagArgs& d, hipStream_t s) {
+  glob_part_k<<<d.rc.kc(), 256, 0, s>>>(d);
+  glob_reduce_k<<<1, 64, 0, s>>>(d);
+  return hipGetLastError();
+}
+
+hipError_t launch_shell_index(const Phys& ph, int* ib, hipStream_t s) {
+  shell_index_k<<<grid_for((size_t)(ph.n / 2 + 1) * ph.n, 256), 256, 0, s>>>(ph, ib);
+  return hipGetLastError();
+}
+
+#endif  // GT_INST_COMMON
+
+}  // namespace gt

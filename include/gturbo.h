@@ -1,0 +1,152 @@
+/*
+ * gturbo.h -- C ABI of libgturbo_hip.so, the MI355X (gfx950) implementation
+ * of Gturbo's pseudospectral time step.
+ *
+ * Every entry point replaces one subroutine or data clause of the reference
+ * (paths relative to the reference checkout) and keeps its argument meaning:
+ * arrays are passed by the address of the caller's host array, exactly as
+ * Fortran passes them to an external subroutine with an implicit interface,
+ * and the library finds the device copy in a present table keyed by that
+ * address -- the role OpenACC's present table plays in the reference
+ * (src/main.f90:67-83).  A field array holds (N+2)*N doubles: the Fortran
+ * array z(N+2,N) in physical space, or complex z(N/2+1,N) in spectral space
+ * (src/modules.f90:4-9).
+ *
+ * Device copies of spectral fields may be kept in an internal layout (the
+ * 2/3-truncated rectangle, ky fastest); they are converted back to the
+ * reference layout by gt_update_host / gt_exit_data / gt_sync_ref and by the
+ * physical-space entry points (gt_fft_inv, gt_produ), never silently lost.
+ *
+ * Every function returns 0 on success or a GT_ERR_* code; gt_last_error()
+ * gives the message.  A context is used by one host thread; all device work
+ * of a context runs in order on the context's own HIP stream.
+ */
+#ifndef GTURBO_H
+#define GTURBO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GT_OK 0
+#define GT_ERR_HIP 1         /* a HIP runtime call failed */
+#define GT_ERR_ARG 2         /* invalid argument or size */
+#define GT_ERR_NOT_PRESENT 3 /* array has no device copy (gt_enter_data) */
+#define GT_ERR_NFORC 4       /* more than NFORC forced modes */
+#define GT_ERR_NOGPU 5       /* no usable gfx950 device */
+
+#define GT_PLAN_D2Z 1 /* cufftPlan2D(plan_dir,NY,NY,CUFFT_D2Z), main.f90:32 */
+#define GT_PLAN_Z2D 2 /* cufftPlan2D(plan_inv,NY,NY,CUFFT_Z2D), main.f90:27 */
+
+typedef struct gt_ctx gt_ctx;
+
+/* --- set-up ------------------------------------------------------------ */
+
+/* Replaces src/inieuler.f90:13-88 (split and RK-order rules, dt/2, dt/3,
+ * dt/6, sqrt(dt)) with src/defk.f90 and src/iniforcing.f90, plus the
+ * streams of src/main.f90:28,33.  params holds the params.dat values in file
+ * order: xlx xly nu alpnu mu alpmu alpv dt rko npas iout imix rtrunc
+ * truncate famp k1f k2f.  n is the compile-time NX=NY of src/modules.f90:4,
+ * a power of two in [16, 8192]. */
+int gt_create(int32_t n, const double *params, gt_ctx **ctx);
+/* Replaces src/main.f90:185-188 (plans and streams destroyed). */
+int gt_destroy(gt_ctx *ctx);
+const char *gt_last_error(void);
+
+/* Scalars after set-up.  iinfo: nf split rko truncate kx p q ;
+ * dinfo: inpz inpe dt2 dt3 dt6 sdt dkx dky rtrunc^2 fkxsmax fkysmax. */
+int gt_info(const gt_ctx *ctx, int32_t *iinfo, double *dinfo);
+/* Host copies of the src/defk.f90 tables (jc is 1-based, emk is
+ * (N/2+1)*N in Fortran order). */
+int gt_get_defk(const gt_ctx *ctx, double *fkx, double *fky, double *fkxs,
+                double *fkys, int32_t *jc, double *emk, double *emkx,
+                double *emky);
+/* The src/iniforcing.f90 table (1-based ikxf, ikyf; ff(2,nf)). */
+int gt_get_forcing_modes(const gt_ctx *ctx, int32_t *ikxf, int32_t *ikyf,
+                         double *ff);
+/* Replaces cufftPlan2D (src/main.f90:27,32): kind GT_PLAN_D2Z/Z2D. */
+int gt_plan_create(gt_ctx *ctx, int32_t kind, int32_t *plan);
+/* Replaces show_mem's cudaMemGetInfo (src/trunc.f90:32-49). */
+int gt_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* --- data clauses (src/main.f90:67-83,112,125,142-145,164-179) --------- */
+
+/* !$acc enter data copyin(a) / create(a): copyin = 1 copies the host array. */
+int gt_enter_data(gt_ctx *ctx, void *host, int64_t bytes, int32_t copyin);
+/* !$acc exit data copyout(a) / delete(a) finalize: copyout = 1 copies back. */
+int gt_exit_data(gt_ctx *ctx, void *host, int32_t copyout);
+/* !$acc update host(a) / update device(a). */
+int gt_update_host(gt_ctx *ctx, void *host);
+int gt_update_device(gt_ctx *ctx, void *host);
+/* acc_map_data: attach caller-owned device memory (e.g. a torch tensor) as
+ * the device copy of key; gt_sync_ref makes that memory current in the
+ * reference layout. */
+int gt_map_data(gt_ctx *ctx, void *key, void *dev, int64_t bytes);
+int gt_unmap_data(gt_ctx *ctx, void *key);
+int gt_sync_ref(gt_ctx *ctx, void *key);
+/* Device-side array copy dst = src, as !$acc kernels present(z,znl);
+ * znl=z (src/main.f90:107-109,120-122). */
+int gt_copy(gt_ctx *ctx, void *dst, const void *src);
+/* Waits for the context's stream. */
+int gt_synchronize(gt_ctx *ctx);
+
+/* --- the hot path ------------------------------------------------------ */
+
+/* src/fft_dir.f90:1-23: in-place D2Z of z, then z/NY/NX. */
+int gt_fft_dir(gt_ctx *ctx, double *z, int32_t plan);
+/* src/fft_inv.f90:1-15: in-place Z2D, unnormalised. */
+int gt_fft_inv(gt_ctx *ctx, double *z, int32_t plan);
+/* src/produ.f90:1-25: u = z*u, v = z*v; padding rows zeroed. */
+int gt_produ(gt_ctx *ctx, const double *z, double *u, double *v);
+/* src/trunc.f90:1-28: 2/3 dealiasing mask and z(1,1) = 0. */
+int gt_trunc(gt_ctx *ctx, double *z);
+/* src/nlt.f90:1-65: z <- N(z) = -ik.(q u)^, truncated when truncate = 1.
+ * u and v are the reference's work arrays; their contents afterwards are
+ * not specified. */
+int gt_nlt(gt_ctx *ctx, double *z, double *u, double *v);
+/* src/rkstep.f90:2-76: integrating-factor RK2 (split or not).  znl, u, v
+ * are work arrays. */
+int gt_step(gt_ctx *ctx, double *z, double *znl, double *u, double *v);
+/* src/rkstep.f90:82-131: Lawson RK4.  znew receives the stage accumulator
+ * (src/rkstep.f90:96-114); znl, u, v are work arrays. */
+int gt_step4(gt_ctx *ctx, double *z, double *znl, double *znew, double *u,
+             double *v);
+/* src/forcing.f90:1-51: adds one draw of the ring forcing and advances the
+ * LCG seed (the seed is kept on the host; the draws are replayed there). */
+int gt_forcing(gt_ctx *ctx, double *z, int32_t *seed);
+
+/* --- diagnostics (src/diagnostics.f90) --------------------------------- */
+
+/* spectrum (:5-56): sp, sz of NBIN = N/2 shells, already divided by dk. */
+int gt_spectrum(gt_ctx *ctx, const double *z, double *sp, double *sz);
+/* fluxes (:62-119): fle, flz (suffix sums over shells). */
+int gt_fluxes(gt_ctx *ctx, const double *z, const double *znl, double *fle,
+              double *flz);
+/* glob (:125-179): out = ee zz disz_nu disz_mu dise_nu dise_mu. */
+int gt_glob(gt_ctx *ctx, const double *z, double *out);
+/* Shell index int(k/dk)+1 of every half-spectrum mode as the device computes
+ * it (diagnostics.f90:31-34), in Fortran order; 0 for the mean mode. */
+int gt_shell_index(gt_ctx *ctx, int32_t *ib);
+
+/* --- measurement ------------------------------------------------------- */
+
+/* Kernel classes timed with HIP events on the context's stream. */
+#define GT_K_YSTART 0 /* y pass: inverse part only */
+#define GT_K_YSTAGE 1 /* y pass: forward + stage update + inverse */
+#define GT_K_YFINAL 2 /* y pass: forward + final update */
+#define GT_K_XPASS 3  /* x pass */
+#define GT_K_FORCING 4
+#define GT_K_OTHER 5
+#define GT_K_NCLASS 6
+int gt_timing_enable(gt_ctx *ctx, int32_t on);
+/* Sums of event-timed kernel durations per class since the last read (ms)
+ * and launch counts; resets the sums. */
+int gt_timing_read(gt_ctx *ctx, double *ms, int32_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,83 @@
+/*
+ * gturbo_oracle.h -- CPU restatement of the reference's pseudospectral time
+ * step.  TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it as the checker.  The product library
+ * (gturbo_amd/csrc) never links, loads or calls it.
+ *
+ * All arrays use the reference layout: a field is (N+2)*N doubles in Fortran
+ * order, z[i + (N+2)*j] in physical space (i = x, fast) and the complex
+ * half-spectrum z[2*(i + (N/2+1)*j) + {0,1}] (i = kx = 0..N/2, j = ky in FFT
+ * order) in spectral space, exactly as src/modules.f90:4-9 and
+ * src/main.f90:14 lay them out.
+ */
+#ifndef GTURBO_ORACLE_H
+#define GTURBO_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_ctx or_ctx;
+
+/* params.dat values in their file order (src/inieuler.f90:14-19):
+ * p[0..1]   xlx xly
+ * p[2..6]   nu alpnu mu alpmu alpv
+ * p[7..8]   dt rko
+ * p[9..11]  npas iout imix
+ * p[12..13] rtrunc truncate
+ * p[14..16] famp k1f k2f
+ * Returns NULL and sets or_last_error() on failure (e.g. NFORC overflow). */
+or_ctx *or_create(int n, const double *p);
+void or_destroy(or_ctx *c);
+const char *or_last_error(void);
+void or_set_threads(int nthreads);
+
+/* Scalars after inieuler: out[0..] = dkx dky dt2 dt3 dt6 sdt rtrunc2
+ * fkxsmax fkysmax inpz inpe ; iout[0..] = nf split rko truncate */
+void or_get_scalars(const or_ctx *c, double *out, int32_t *iout);
+/* Tables from defk (src/defk.f90): fkx,fkxs (N/2+1), fky,fkys (N), jc (N,
+ * 1-based like the reference), emk (N/2+1)*N, emkx (N/2+1), emky (N). */
+void or_get_defk(const or_ctx *c, double *fkx, double *fky, double *fkxs,
+                 double *fkys, int32_t *jc, double *emk, double *emkx,
+                 double *emky);
+/* Forced-mode table from iniforcing (1-based indices, as the reference). */
+void or_get_forcing_modes(const or_ctx *c, int32_t *ikxf, int32_t *ikyf,
+                          double *ff);
+
+/* The reference subroutines. */
+float or_rann(int32_t *seed);
+void or_fft_dir(const or_ctx *c, double *z);
+void or_fft_inv(const or_ctx *c, double *z);
+void or_produ(const or_ctx *c, const double *z, double *u, double *v);
+void or_trunc(const or_ctx *c, double *z);
+void or_nlt(const or_ctx *c, double *z, double *u, double *v);
+void or_step(const or_ctx *c, double *z, double *znl, double *u, double *v);
+void or_step4(const or_ctx *c, double *z, double *znl, double *znew,
+              double *u, double *v);
+void or_forcing(const or_ctx *c, double *z, int32_t *seed);
+/* Forcing phases for nsteps calls of forcing(): cs[2*(s*nf+k)+{0,1}] =
+ * (cos te, sin te); seed advances as nsteps*nf draws of rann. */
+void or_forcing_phases(const or_ctx *c, int32_t *seed, int nsteps,
+                       double *cs);
+void or_spectrum(const or_ctx *c, const double *z, double *sp, double *sz);
+void or_fluxes(const or_ctx *c, const double *z, const double *znl,
+               double *fle, double *flz);
+/* out = ee zz disz_nu disz_mu dise_nu dise_mu */
+void or_glob(const or_ctx *c, const double *z, double *out);
+/* Shell index ib (1-based) of every half-spectrum mode, ib[i+(N/2+1)*j];
+ * 0 for the mean mode that the reference skips. */
+void or_shell_index(const or_ctx *c, int32_t *ib);
+
+/* The driver loop of src/main.f90:89-137 without file I/O: z holds the
+ * physical input field on entry and the physical output field on return
+ * (main.f90:143).  Every imix steps the diagnostics of main.f90:106-117 are
+ * appended to diag (per call: NBIN sp, NBIN sz, NBIN fle, NBIN flz, 6 glob)
+ * when diag is not NULL.  t is advanced by dt per step (main.f90:98). */
+void or_run(const or_ctx *c, double *z, int32_t *seed, int npas, double *t,
+            double *diag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,217 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same
+seeded inputs.  Bars: integer/index results bit-exact; FP64 fields within the
+tolerances written in each test (1e-10 relative L2 for the vorticity after a
+run, 1e-9 for spectra and fluxes, tighter for single operators)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from helpers import params_for, random_physical, random_spectral, as_complex
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [16, 32, 64, 128, 256, 512]
+
+
+def _pair(oracle_mod, p):
+    from gturbo_amd.solver import Solver
+    return Solver(p), oracle_mod.Oracle(p)
+
+
+def _present(s, *arrays):
+    for a in arrays:
+        s.enter_data(a)
+
+
+@pytest.mark.parametrize("n", SIZES + [1024, 4096])
+def test_fft_dir_matches_oracle(oracle_mod, n):
+    p = params_for(n)
+    s, o = _pair(oracle_mod, p)
+    z = random_physical(n, 10 + n)
+    zg = z.copy()
+    s.enter_data(zg)
+    s.fft_dir(zg)
+    s.update_host(zg)
+    o.fft_dir(z)
+    assert rel_l2(zg, z) < 1e-14
+
+
+@pytest.mark.parametrize("n", SIZES + [1024, 4096])
+def test_fft_inv_matches_oracle(oracle_mod, n):
+    p = params_for(n)
+    s, o = _pair(oracle_mod, p)
+    z = random_spectral(n, 20 + n, truncated=False)
+    zg = z.copy()
+    s.enter_data(zg)
+    s.fft_inv(zg)
+    s.update_host(zg)
+    o.fft_inv(z)
+    assert rel_l2(zg, z) < 1e-14
+
+
+@pytest.mark.parametrize("n", [16, 64, 256])
+def test_trunc_and_produ(oracle_mod, n):
+    p = params_for(n)
+    s, o = _pair(oracle_mod, p)
+    z = random_spectral(n, 3, truncated=False)
+    zg = z.copy()
+    s.enter_data(zg)
+    s.trunc(zg)
+    s.update_host(zg)
+    o.trunc(z)
+    assert np.array_equal(zg, z)
+    q, u, v = (random_physical(n, k) for k in (1, 2, 3))
+    qg, ug, vg = q.copy(), u.copy(), v.copy()
+    _present(s, qg, ug, vg)
+    s.produ(qg, ug, vg)
+    s.update_host(ug)
+    s.update_host(vg)
+    o.produ(q, u, v)
+    assert np.array_equal(ug, u) and np.array_equal(vg, v)
+
+
+@pytest.mark.parametrize("n", SIZES + [1024])
+@pytest.mark.parametrize("alpv", [0.5, 1.0, 2.0])
+def test_nlt_matches_oracle(oracle_mod, n, alpv):
+    p = params_for(n, alpv=alpv)
+    s, o = _pair(oracle_mod, p)
+    z = random_spectral(n, 7 * n + int(alpv * 10))
+    zg, ug, vg = z.copy(), s.field(), s.field()
+    _present(s, zg, ug, vg)
+    s.nlt(zg, ug, vg)
+    s.update_host(zg)
+    o.nlt(z)
+    assert rel_l2(zg, z) < 1e-12
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_nlt_untruncated_input(oracle_mod, n):
+    """A spectrum with modes outside the 2/3 rectangle takes the full-rectangle
+    path and still matches."""
+    p = params_for(n, alpv=1.0)
+    s, o = _pair(oracle_mod, p)
+    z = random_spectral(n, 5, truncated=False)
+    zg, ug, vg = z.copy(), s.field(), s.field()
+    _present(s, zg, ug, vg)
+    s.nlt(zg, ug, vg)
+    s.update_host(zg)
+    o.nlt(z)
+    assert rel_l2(zg, z) < 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 64, 256])
+@pytest.mark.parametrize("truncate", [0, 1])
+def test_nlt_truncate_flag(oracle_mod, n, truncate):
+    p = params_for(n, alpv=2.0, truncate=truncate)
+    s, o = _pair(oracle_mod, p)
+    z = random_spectral(n, 11, truncated=bool(truncate))
+    zg, ug, vg = z.copy(), s.field(), s.field()
+    _present(s, zg, ug, vg)
+    s.nlt(zg, ug, vg)
+    s.update_host(zg)
+    o.nlt(z)
+    assert rel_l2(zg, z) < 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 64, 256, 512])
+@pytest.mark.parametrize("rko", [2, 4])
+def test_step_matches_oracle(oracle_mod, n, rko):
+    p = params_for(n, alpv=1.0, rko=rko)
+    s, o = _pair(oracle_mod, p)
+    z = random_spectral(n, 100 + n)
+    zg = z.copy()
+    scratch = [s.field() for _ in range(4)]
+    _present(s, zg, *scratch)
+    for _ in range(3):
+        if rko == 4:
+            s.step4(zg, *scratch)
+            o.step4(z)
+        else:
+            s.step(zg, scratch[0], scratch[2], scratch[3])
+            o.step(z)
+    s.update_host(zg)
+    assert rel_l2(zg, z) < 1e-12
+
+
+@pytest.mark.parametrize("n", [32, 128])
+def test_split_rk2_matches_oracle(oracle_mod, n):
+    """alpnu = 1, alpmu = 0, rko = 2 selects the split integrating factor
+    emkx*emky (src/inieuler.f90:24-36, src/rkstep.f90:21-27)."""
+    p = params_for(n, alpv=2.0, rko=2, alpnu=1.0, nu=1e-3, alpmu=0.0)
+    s, o = _pair(oracle_mod, p)
+    assert s.split == 1 and o.split == 1
+    z = random_spectral(n, 9)
+    zg = z.copy()
+    scratch = [s.field() for _ in range(3)]
+    _present(s, zg, *scratch)
+    for _ in range(3):
+        s.step(zg, *scratch)
+        o.step(z)
+    s.update_host(zg)
+    assert rel_l2(zg, z) < 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 64, 256, 1024])
+def test_forcing_bit_exact(oracle_mod, n):
+    p = params_for(n)
+    s, o = _pair(oracle_mod, p)
+    assert s.nf == o.nf and s.nf > 0
+    z = random_spectral(n, 4)
+    zg = z.copy()
+    s.enter_data(zg)
+    seed_g = seed_o = 481929
+    for _ in range(5):
+        seed_g = s.forcing(zg, seed_g)
+        seed_o = o.forcing(z, seed_o)
+    s.update_host(zg)
+    assert seed_g == seed_o
+    assert np.array_equal(zg, z)
+
+
+@pytest.mark.parametrize("n", [16, 256, 4096])
+def test_shell_index_bit_exact(oracle_mod, n):
+    p = params_for(n)
+    s, o = _pair(oracle_mod, p)
+    assert np.array_equal(s.shell_index(), o.shell_index())
+
+
+@pytest.mark.parametrize("n", [16, 64, 256, 1024])
+@pytest.mark.parametrize("alpv", [1.0, 2.0])
+def test_diagnostics_match_oracle(oracle_mod, n, alpv):
+    p = params_for(n, alpv=alpv)
+    s, o = _pair(oracle_mod, p)
+    z = random_spectral(n, 55 + n)
+    znl = z.copy()
+    o.nlt(znl)
+    zg, ng = z.copy(), znl.copy()
+    _present(s, zg, ng)
+    sp, sz = s.spectrum(zg)
+    osp, osz = o.spectrum(z)
+    assert rel_l2(sp, osp) < 1e-12 and rel_l2(sz, osz) < 1e-12
+    fle, flz = s.fluxes(zg, ng)
+    ofle, oflz = o.fluxes(z, znl)
+    scale_e = np.abs(ofle).max() + 1e-300
+    scale_z = np.abs(oflz).max() + 1e-300
+    assert np.abs(fle - ofle).max() / scale_e < 1e-9
+    assert np.abs(flz - oflz).max() / scale_z < 1e-9
+    g, og = s.glob(zg), o.glob(z)
+    assert np.allclose(g, og, rtol=1e-12, atol=0)
+
+
+def test_run_euler256_matches_oracle(oracle_mod):
+    """The 256^2 Euler workload (alpha=2, RK4, dt=1e-3, nu=1e-16 k^8, mu=0.1,
+    ring (15,17]) over its full 1000 steps from the same float32 initial field
+    and seed: vorticity within 1e-10 relative L2 of the oracle."""
+    from gturbo_amd import ic, io, workloads
+    w = workloads()["euler256"]
+    p = w.params
+    s, o = _pair(oracle_mod, p)
+    w0 = ic.physical_field(p, w.ic)
+    zg = io.field_to_ref(w0)
+    zo = zg.copy()
+    seed_g, _ = s.run(zg, w.forcing_seed, p.npas)
+    seed_o, _, _ = o.run(zo, w.forcing_seed, p.npas)
+    assert seed_g == seed_o
+    err = rel_l2(zg[:, :p.n], zo[:, :p.n])
+    print(f"euler256 1000-step rel L2 = {err:.3e}")
+    assert err < 1e-10
