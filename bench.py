@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Headline benchmark: RK4 grid-point-steps/s of the pseudospectral step at
+4096^2 FP64 (N=4096, alpha=2, dt=5e-5, nu=1e-28 k^8, mu=0.02, ring (63,65],
+initial field from seed 2 with spectrum k^4 exp(-(k/32)^2)).
+
+A step is what the reference's driver loop does per iteration when no
+diagnostics or output are due (src/main.f90:97-137): step4 + forcing, on a
+state resident in HBM.  W warm-up steps, then K timed steps bracketed by
+device synchronisation.  Per-kernel durations come from HIP events recorded on
+the library's own stream over the timed region; the roofline entry is the
+kernel class with the largest total time, priced at its algorithmic bytes.
+The CPU baseline is the oracle (plain C, OpenMP) timed on this host on a
+bounded sample of the same workload.
+
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RK4 grid-pt-steps/s at 4096² FP64; ensemble & adjoint vs plain loop"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(n, rect, rko):
+    """HBM bytes each kernel class must move per launch on the compact layout
+    (a kept mode is 16 B; a y-buffer field holds N*kc complex values; emk is
+    8 B per kept mode)."""
+    kx, p, q = rect
+    kc, ky = kx + 1, p + 1 + q
+    Z = 16.0 * kc * ky          # one compact spectral field
+    E = 8.0 * kc * ky           # the integrating-factor table
+    Y = 16.0 * n * kc           # one field between the passes
+    if rko == 4:
+        stage = 5 * Y + E + (2 * Z + 3 * Z + 3 * Z) / 3.0  # stages 1-3 averaged
+        final = 2 * Y + 2 * Z
+    else:
+        stage = 5 * Y + E + Z
+        final = 2 * Y + E + 2 * Z
+    return {"y_start": Z + 3 * Y, "y_stage": stage, "y_final": final,
+            "x_pass": 5 * Y}
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE), from the committed summary, or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_euler4096_rk4.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch":
+                vals[parts[1]] = float(parts[2])
+    return vals.get(kernel)
+
+
+def cpu_baseline(p, w0, seed, budget_s=12.0, max_steps=4):
+    import oracle
+    oracle.build()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    oracle.set_threads(threads)
+    o = oracle.Oracle(p)
+    from gturbo_amd import io
+    z = io.field_to_ref(w0)
+    o.fft_dir(z)
+    o.trunc(z)
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps:
+        if p.rko == 4:
+            o.step4(z)
+        else:
+            o.step(z)
+        seed = o.forcing(z, seed)
+        steps += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": p.n * p.n * steps / el, "unit": "grid-pt-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} step(s)+forcing of the same {p.n}^2 RK{p.rko} workload "
+                      f"({el:.1f} s wall, {threads} OpenMP threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--rko", type=int, default=4, choices=(2, 4))
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one device-to-device copy of a reference-layout "
+                         "field (a plain copy of known size for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the 2D pseudospectral "
+                          "step is not sharded across GPUs (see DESIGN.md); "
+                          f"--gpus {args.gpus} is not supported"}))
+        sys.exit(2)
+
+    from gturbo_amd import ic, io, workloads
+    from gturbo_amd.solver import Solver
+    w = workloads()[f"euler4096_rk{args.rko}"]
+    p = w.params.replace(n=args.n)
+    w0 = ic.physical_field(p, w.ic)
+    s = Solver(p)
+    z = io.field_to_ref(w0)
+    znl, znew, u, v = (s.field() for _ in range(4))
+    s.enter_data(z)
+    for a in (znl, znew, u, v):
+        s.enter_data(a, copyin=False)
+    if args.calib_copy:
+        # (N+2)*N*8 bytes read and written by the runtime's copy kernel
+        s.copy(znl, u)
+        s.synchronize()
+    s.fft_dir(z)
+    s.trunc(z)
+    seed = w.forcing_seed
+
+    def one_step(seed):
+        if p.rko == 4:
+            s.step4(z, znl, znew, u, v)
+        else:
+            s.step(z, znl, u, v)
+        return s.forcing(z, seed)
+
+    for _ in range(args.warmup):
+        seed = one_step(seed)
+    s.synchronize()
+    s.timing_read()
+    s.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        seed = one_step(seed)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    s.timing_enable(False)
+    kt = s.timing_read()
+    # the state must stay finite
+    s.update_host(z)
+    if not np.all(np.isfinite(z)):
+        print(json.dumps({"error": "non-finite state after the timed steps"}))
+        sys.exit(3)
+
+    rate = p.n * p.n * args.steps / el
+    ab = algorithmic_bytes(p.n, s.rect, p.rko)
+    kernels = {}
+    for name, (ms, cnt) in kt.items():
+        if cnt == 0:
+            continue
+        avg_s = ms / cnt / 1e3
+        ent = {"launches": cnt, "avg_us": avg_s * 1e6, "total_ms": ms}
+        if name in ab:
+            ent["algorithmic_bytes"] = ab[name]
+            ent["achieved_GBs"] = ab[name] / avg_s / 1e9
+        kernels[name] = ent
+    dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["total_ms"])
+    ach = kernels[dom]["achieved_GBs"]
+    traffic = read_traffic(dom)
+    roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic}
+    step_bytes = sum(ab[k] * kernels[k]["launches"] for k in ab if k in kernels) / args.steps
+    out = {
+        "metric": METRIC,
+        "value": rate,
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": w.name if args.n == 4096 else f"{w.name} at N={args.n}",
+                   "n": p.n, "rko": p.rko, "dt": p.dt, "alpv": p.alpv,
+                   "kept_rect": list(s.rect), "nf": s.nf},
+        "roofline": roof,
+        "step_model": {"algorithmic_bytes_per_step": step_bytes,
+                       "bytes_per_gp_step": step_bytes / (p.n * p.n),
+                       "hbm_GBs_at_measured_rate": step_bytes * args.steps / el / 1e9},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(p, w0, w.forcing_seed)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libgturbo_hip.so")
+LIB_PATH = os.environ.get("GTURBO_LIB") or os.path.join(HERE, "libgturbo_hip.so")
 
 # Every symbol include/gturbo.h declares, with (restype, argtypes).
 _P = C.c_void_p

@@ -1,20 +1,28 @@
 // fft_core.hpp -- FP64 complex FFT building blocks for gfx950.
 //
-// A line of N = 2^LOGN complex doubles lives in LDS (padded: element e sits
-// at e + e/16, which spreads the stride-16 writes of the first pass over the
-// banks).  TPL = N/16 threads own one line.  Each Stockham pass reads 16
-// elements per thread at stride N/R, multiplies by twiddles, runs a radix-R
-// DFT in registers and writes them back at stride Ns, so the result comes
-// out in natural order with no bit reversal.  N = 16^a * r uses a radix-16
-// passes and one radix-r pass (r in {2,4,8}).
+// A line of N = 2^LOGN complex doubles is spread over TPL = N/8 threads:
+// thread j holds a[r] = x[j + TPL*r], r = 0..7, in registers, before and
+// after the transform.  The transform is a Stockham sequence of radix-8
+// passes (plus one radix-2 or radix-4 pass when log2 N is not a multiple of
+// 3).  Every pass reads exactly that per-thread set; the last pass
+// (Ns*R = N) writes it back in natural order, so only the transposes between
+// passes go through LDS (one padded line per transform: element e sits at
+// e + e/16, which spreads the strided writes over the banks).
+//
+// Eight elements per thread keep a transform at 32 VGPRs of data per lane,
+// which lets the passes that carry a second field run at 4 waves per SIMD.
 //
 // Twiddles come from a table W[m] = exp(-2 pi i m / N), m < N, built on the
-// host in extended precision and rounded once, so every twiddle is the
-// correctly rounded value.
+// host in extended precision and rounded once.  A radix-8 butterfly loads
+// W^k, W^2k, W^4k and forms W^3k, W^5k, W^6k, W^7k by complex products (at
+// most two extra roundings), which keeps three twiddles live instead of
+// seven.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace gt {
+
+constexpr int kE = 8;  // complex elements per thread and line
 
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) {
   return make_double2(a.x + b.x, a.y + b.y);
@@ -35,25 +43,19 @@ __device__ __forceinline__ double2 mul_is(double2 a) {
   return S > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
 
-constexpr double kC8 = 0.92387953251128675613;  // cos(pi/8)
-constexpr double kS8 = 0.38268343236508977173;  // sin(pi/8)
 constexpr double kR2 = 0.70710678118654752440;  // cos(pi/4)
 
-// a * exp(S*2*pi*i*M/16)
+// a * exp(S*2*pi*i*M/8)
 template <int S, int M>
-__device__ __forceinline__ double2 w16(double2 a) {
-  constexpr int m = M & 15;
+__device__ __forceinline__ double2 w8(double2 a) {
+  constexpr int m = M & 7;
   if constexpr (m == 0) return a;
-  else if constexpr (m == 4) return mul_is<S>(a);
-  else if constexpr (m == 8) return make_double2(-a.x, -a.y);
-  else if constexpr (m == 12) return mul_is<-S>(a);
+  else if constexpr (m == 2) return mul_is<S>(a);
+  else if constexpr (m == 4) return make_double2(-a.x, -a.y);
+  else if constexpr (m == 6) return mul_is<-S>(a);
   else {
-    constexpr double c = (m == 1 || m == 15) ? kC8 : (m == 2 || m == 14) ? kR2
-                       : (m == 3 || m == 13) ? kS8 : (m == 5 || m == 11) ? -kS8
-                       : (m == 6 || m == 10) ? -kR2 : -kC8;
-    constexpr double s0 = (m == 1 || m == 7) ? kS8 : (m == 2 || m == 6) ? kR2
-                        : (m == 3 || m == 5) ? kC8 : (m == 9 || m == 15) ? -kS8
-                        : (m == 10 || m == 14) ? -kR2 : -kC8;
+    constexpr double c = (m == 1 || m == 7) ? kR2 : -kR2;
+    constexpr double s0 = (m == 1 || m == 3) ? kR2 : -kR2;
     constexpr double s = S > 0 ? s0 : -s0;
     return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
   }
@@ -82,9 +84,9 @@ template <int S>
 __device__ __forceinline__ void dft8(double2* a) {
   dft4<S>(a[0], a[2], a[4], a[6]);
   dft4<S>(a[1], a[3], a[5], a[7]);
-  a[3] = w16<S, 2>(a[3]);
-  a[5] = w16<S, 4>(a[5]);
-  a[7] = w16<S, 6>(a[7]);
+  a[3] = w8<S, 1>(a[3]);
+  a[5] = w8<S, 2>(a[5]);
+  a[7] = w8<S, 3>(a[7]);
   double2 y[8];
 #pragma unroll
   for (int k1 = 0; k1 < 4; ++k1) {
@@ -95,105 +97,94 @@ __device__ __forceinline__ void dft8(double2* a) {
   for (int i = 0; i < 8; ++i) a[i] = y[i];
 }
 
-// Radix-16 as 4 x 4 (n = 4 n1 + n2, k = k1 + 4 k2), natural-order output.
-template <int S>
-__device__ __forceinline__ void dft16(double2* a) {
-  dft4<S>(a[0], a[4], a[8], a[12]);
-  dft4<S>(a[1], a[5], a[9], a[13]);
-  dft4<S>(a[2], a[6], a[10], a[14]);
-  dft4<S>(a[3], a[7], a[11], a[15]);
-  // a[4*k1 + n2] *= W16^(n2*k1)
-  a[5] = w16<S, 1>(a[5]);
-  a[9] = w16<S, 2>(a[9]);
-  a[13] = w16<S, 3>(a[13]);
-  a[6] = w16<S, 2>(a[6]);
-  a[10] = w16<S, 4>(a[10]);
-  a[14] = w16<S, 6>(a[14]);
-  a[7] = w16<S, 3>(a[7]);
-  a[11] = w16<S, 6>(a[11]);
-  a[15] = w16<S, 9>(a[15]);
-  dft4<S>(a[0], a[1], a[2], a[3]);
-  dft4<S>(a[4], a[5], a[6], a[7]);
-  dft4<S>(a[8], a[9], a[10], a[11]);
-  dft4<S>(a[12], a[13], a[14], a[15]);
-  // y[k1 + 4 k2] sits at a[4 k1 + k2]: transpose the 4x4 block.
-  double2 y[16];
-#pragma unroll
-  for (int k1 = 0; k1 < 4; ++k1)
-#pragma unroll
-    for (int k2 = 0; k2 < 4; ++k2) y[k1 + 4 * k2] = a[4 * k1 + k2];
-#pragma unroll
-  for (int i = 0; i < 16; ++i) a[i] = y[i];
-}
-
 template <int R, int S>
 __device__ __forceinline__ void dft(double2* a) {
   if constexpr (R == 2) dft2<S>(a[0], a[1]);
   else if constexpr (R == 4) dft4<S>(a[0], a[1], a[2], a[3]);
-  else if constexpr (R == 8) dft8<S>(a);
-  else dft16<S>(a);
+  else dft8<S>(a);
 }
 
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
 
-// Elements of LDS needed by one padded line.
 template <int LOGN>
 struct Line {
   static constexpr int N = 1 << LOGN;
-  static constexpr int TPL = N / 16;          // threads per line
+  static constexpr int TPL = N / kE;          // threads per line
   static constexpr int PADDED = N + N / 16;   // double2 per line in LDS
-  static constexpr int NPASS16 = LOGN / 4;    // radix-16 passes
-  static constexpr int RLAST = 1 << (LOGN % 4);  // trailing radix (1 = none)
+  static constexpr int NPASS8 = LOGN / 3;     // radix-8 passes
+  static constexpr int RLAST = 1 << (LOGN % 3);  // trailing radix (1 = none)
 };
 
-// One Stockham pass of radix R with stride Ns over a padded LDS line.
-template <int LOGN, int S, int R, int NS>
-__device__ __forceinline__ void stockham_pass(double2* buf, int j,
-                                              const double2* __restrict__ tw) {
+// One Stockham pass: radix R, stride Ns.  Butterfly jj (< N/R) reads
+// x[jj + (N/R) r'] and writes y[(jj/Ns) Ns R + jj%Ns + Ns r'].
+template <int LOGN, int S, int R, int NS, bool LAST>
+__device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
+                                         const double2* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
-  constexpr int TPL = N / 16;
-  constexpr int NB = 16 / R;  // butterflies per thread
-  double2 a[16];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int jj = j + TPL * b;
-#pragma unroll
-    for (int r = 0; r < R; ++r) a[b * R + r] = buf[lpad(jj + r * (N / R))];
-  }
-  __syncthreads();
+  constexpr int TPL = N / kE;
+  constexpr int NB = kE / R;  // butterflies per thread
+  // Opaque copy of the thread index: the addresses below are then computed
+  // inside each pass instead of being shared (and kept live) across every
+  // transform of the kernel.
+  asm volatile("" : "+v"(j));
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     const int jj = j + TPL * b;
     const int k = jj % NS;
+    double2 t[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) t[r] = a[b + NB * r];
     if constexpr (NS > 1) {
-#pragma unroll
-      for (int r = 1; r < R; ++r) {
-        double2 w = tw[r * k * (N / (NS * R))];
-        a[b * R + r] = S < 0 ? cmul(a[b * R + r], w) : cmulc(a[b * R + r], w);
+      // W^(r k), r < R, from the table entries for r = 1, 2, 4
+      constexpr int ST = N / (NS * R);
+      double2 w[R];
+      w[1] = tw[k * ST];
+      if constexpr (R >= 4) {
+        w[2] = tw[2 * k * ST];
+        w[3] = cmul(w[1], w[2]);
       }
-    }
-    dft<R, S>(a + b * R);
-    const int idx = (jj / NS) * NS * R + k;
+      if constexpr (R == 8) {
+        w[4] = tw[4 * k * ST];
+        w[5] = cmul(w[1], w[4]);
+        w[6] = cmul(w[2], w[4]);
+        w[7] = cmul(w[3], w[4]);
+      }
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[lpad(idx + r * NS)] = a[b * R + r];
+      for (int r = 1; r < R; ++r) t[r] = S < 0 ? cmul(t[r], w[r]) : cmulc(t[r], w[r]);
+    }
+    dft<R, S>(t);
+    if constexpr (LAST) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[b + NB * r] = t[r];
+    } else {
+      const int idx = (jj / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[lpad(idx + r * NS)] = t[r];
+    }
   }
-  __syncthreads();
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kE; ++r) a[r] = buf[lpad(j + TPL * r)];
+    __syncthreads();
+  }
 }
 
-// Full in-LDS transform of one line; S = -1 forward, +1 inverse
-// (unnormalised).  Every thread of the workgroup must call it (it holds
-// barriers); j is the thread's index within its line.
+// Transform of one line held in registers; S = -1 forward, +1 inverse
+// (unnormalised).  Every thread of the workgroup must call it (the
+// transposes hold barriers); j is the thread's index within its line.
 template <int LOGN, int S>
-__device__ __forceinline__ void fft_line(double2* buf, int j,
-                                         const double2* __restrict__ tw) {
+__device__ __forceinline__ void fft_reg(double2* a, double2* buf, int j,
+                                        const double2* __restrict__ tw) {
   using L = Line<LOGN>;
-  if constexpr (L::NPASS16 >= 1) stockham_pass<LOGN, S, 16, 1>(buf, j, tw);
-  if constexpr (L::NPASS16 >= 2) stockham_pass<LOGN, S, 16, 16>(buf, j, tw);
-  if constexpr (L::NPASS16 >= 3) stockham_pass<LOGN, S, 16, 256>(buf, j, tw);
-  if constexpr (L::NPASS16 >= 4) stockham_pass<LOGN, S, 16, 4096>(buf, j, tw);
+  constexpr int NP = L::NPASS8 + (L::RLAST > 1 ? 1 : 0);
+  if constexpr (L::NPASS8 >= 1) reg_pass<LOGN, S, 8, 1, NP == 1>(a, buf, j, tw);
+  if constexpr (L::NPASS8 >= 2) reg_pass<LOGN, S, 8, 8, NP == 2>(a, buf, j, tw);
+  if constexpr (L::NPASS8 >= 3) reg_pass<LOGN, S, 8, 64, NP == 3>(a, buf, j, tw);
+  if constexpr (L::NPASS8 >= 4) reg_pass<LOGN, S, 8, 512, NP == 4>(a, buf, j, tw);
   if constexpr (L::RLAST > 1) {
-    constexpr int NS = 1 << (4 * L::NPASS16);
-    stockham_pass<LOGN, S, L::RLAST, NS>(buf, j, tw);
+    constexpr int NS = 1 << (3 * L::NPASS8);
+    reg_pass<LOGN, S, L::RLAST, NS, true>(a, buf, j, tw);
   }
 }
 

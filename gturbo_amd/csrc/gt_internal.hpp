@@ -26,6 +26,7 @@ struct Phys {
   double rtrunc2, fkxsmax, fkysmax;
   int truncate;
   double inv_n;  // 1/N (fft_dir.f90:19 divides by NY then NX)
+  double inv_n2; // 1/N^2, exact for N = 2^k
   double nu, mu, alpnu, alpmu, alpv;
 };
 
@@ -55,11 +56,16 @@ enum CMode : int {
   C_RK2_2 = 7,
 };
 
-// Column-tile width of the y pass and row pairs per workgroup of the x pass,
-// per log2 N.  The y-pass LDS is BT lines of (N + N/16) double2.
+// Column-tile width of the y pass (BT), y rows per workgroup of the x pass,
+// and row pairs per workgroup of the reference-layout transforms, per
+// log2 N.  A line is N/8 threads and (N + N/16) double2 of LDS.
 __host__ __device__ constexpr int bt_of(int logn) {
   return logn <= 6 ? 64 : logn == 7 ? 32 : logn == 8 ? 16 : logn == 9 ? 8
        : logn == 10 ? 4 : logn <= 12 ? 2 : 1;
+}
+__host__ __device__ constexpr int rows_of(int logn) {
+  return logn <= 6 ? (1 << logn) : logn == 7 ? 32 : logn == 8 ? 16
+       : logn == 9 ? 8 : logn == 10 ? 4 : logn <= 12 ? 2 : 1;
 }
 __host__ __device__ constexpr int rp_of(int logn) {
   return logn <= 6 ? (1 << (logn - 1)) : logn == 7 ? 32 : logn == 8 ? 16

@@ -8,14 +8,16 @@
 //    truncation and the RK stage update (src/rkstep.f90 stage AXPYs).  Inverse
 //    part: from the next stage input, the spectral operator |k|^-alpha
 //    (src/nlt.f90:22-36) for q, u, v and their inverse y transforms.
-//  * x pass (r_pass): one workgroup per RP pairs of y rows.  Complex-to-real
-//    x transforms of q, u, v (two rows per complex transform), the products
-//    q u, q v (src/produ.f90), and the real-to-complex x transforms of the
-//    products (again two rows per transform).
+//  * x pass (r_pass): one workgroup per RP y rows.  Complex-to-real x
+//    transforms of q+iv and q+iu (two real sequences per complex transform),
+//    the products q u, q v (src/produ.f90), and one real-to-complex transform
+//    of q u + i q v.
 //
 // Between the passes the three (resp. two) fields live in HBM in a tiled
 // layout [field][tile][y][kx % BT]: the y pass reads and writes whole tiles
-// contiguously, the x pass reads and writes 2*BT*16-byte runs.
+// contiguously; the x pass reads and writes BT*16-byte runs, and the
+// workgroup order keeps neighbouring rows on one XCD so that their runs
+// share L2 lines.
 #include "fft_core.hpp"
 #include "gt_internal.hpp"
 
@@ -36,6 +38,23 @@ __device__ __forceinline__ bool cut_mode(const Phys& ph, double fkxs,
 __device__ __forceinline__ double kalpha(const Phys& ph, double k2) {
   // ka = k2**(alpv/2) (src/nlt.f90:27)
   return ph.ahalf_is_one ? k2 : pow(k2, ph.ahalf);
+}
+
+// ka for a kernel specialised on the alpha kind: AK = 1 when alpv/2 = 1
+// (ka = k2 exactly, no pow code in the kernel), else the general power.
+template <int AK>
+__device__ __forceinline__ double kalpha_t(const Phys& ph, double k2) {
+  if constexpr (AK == 1) return k2;
+  else return pow(k2, ph.ahalf);
+}
+
+// With the 2/3 rule the kept rectangle stays below |k| = 3N/8, so of a
+// thread's positions p = j + r*N/8 the ones with r = 3, 4 are never kept:
+// those inputs are zero and those outputs unused.  TR = 1 makes that a
+// compile-time fact.
+template <int TR>
+__device__ __forceinline__ constexpr bool live_r(int r) {
+  return TR == 0 || (r != 3 && r != 4);
 }
 
 template <int LOGN, int BT>
@@ -72,16 +91,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
 // y pass
 // ---------------------------------------------------------------------------
 
-template <int LOGN, int MODE>
-__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
+template <int LOGN, int MODE, int AK, int TR>
+__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 1)
     c_pass(PassArgs a) {
-  constexpr int N = 1 << LOGN, TPL = N / 16, BT = bt_of(LOGN);
+  constexpr int N = 1 << LOGN, TPL = N / kE, BT = bt_of(LOGN);
   constexpr int LINE = Line<LOGN>::PADDED;
   constexpr bool FWD = MODE != C_START;
   constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
                        MODE == C_RK4_3 || MODE == C_RK2_1;
   __shared__ double2 lds[BT * LINE];
-  const int tid = threadIdx.x, b = tid / TPL, j = tid % TPL;
+  // Columns interleave over the lanes so that a tile row (BT columns at one
+  // y) is read or written by adjacent lanes.
+  const int tid = threadIdx.x, b = tid % BT, j = tid / BT;
   const int tile = blockIdx.x;
   const int c = tile * BT + b;
   const Rect rc = a.rc;
@@ -90,41 +111,41 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
   const Phys& ph = a.ph;
   const double fkx = (double)c * ph.dkx;
   const double fkxs = fkx * fkx;
-  const size_t cbase = (size_t)c * a.kyp;
-  double2 s[16];
+  const unsigned cbase = (unsigned)c * (unsigned)a.kyp;
+  double2 s[kE];
+  double2 z[kE];
 
   if constexpr (FWD) {
-    double2 A[16];
-    load_tile<LOGN, BT>(lds, a.y2 + (size_t)tile * N * BT, tid);
-    __syncthreads();
-    fft_line<LOGN, -1>(buf, j, a.tw);
+    double2 A[kE];
+    const double2* srcA = a.y2 + (size_t)tile * N * BT;
 #pragma unroll
-    for (int m = 0; m < 16; ++m) A[m] = buf[lpad(j + TPL * m)];
-    __syncthreads();
-    load_tile<LOGN, BT>(lds, a.y2 + (size_t)(a.ntiles + tile) * N * BT, tid);
-    __syncthreads();
-    fft_line<LOGN, -1>(buf, j, a.tw);
+    for (int r = 0; r < kE; ++r) z[r] = srcA[(unsigned)((j + TPL * r) * BT + b)];
+    fft_reg<LOGN, -1>(z, buf, j, a.tw);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int p = j + TPL * m;
-      const int r = row_of(p, rc, N);
-      double2 Bv = buf[lpad(p)];
-      s[m] = make_double2(0.0, 0.0);
-      if (colok && r >= 0) {
+    for (int r = 0; r < kE; ++r) A[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
+    const double2* srcB = a.y2 + (size_t)(a.ntiles + tile) * N * BT;
+#pragma unroll
+    for (int r = 0; r < kE; ++r) z[r] = srcB[(unsigned)((j + TPL * r) * BT + b)];
+    fft_reg<LOGN, -1>(z, buf, j, a.tw);
+    int je = j;
+    // The epilogue's indices (and so its loads) depend on the transform's
+    // output: nothing of the epilogue is scheduled inside the last pass.
+    asm volatile("" : "+v"(je) : "v"(z[0].x), "v"(z[kE - 1].y));
+#pragma unroll
+    for (int r = 0; r < kE; ++r) {
+      const int p = je + TPL * r;
+      const int row = row_of(p, rc, N);
+      s[r] = make_double2(0.0, 0.0);
+      if (live_r<TR>(r) && colok && row >= 0) {
         const double fky = (double)kyi_of(p, N) * ph.dky;
         const double fkys = fky * fky;
-        // fft_dir normalisation (src/fft_dir.f90:19), exact for N = 2^k
-        double2 Av = A[m];
-        Av.x = Av.x * ph.inv_n * ph.inv_n;
-        Av.y = Av.y * ph.inv_n * ph.inv_n;
-        Bv.x = Bv.x * ph.inv_n * ph.inv_n;
-        Bv.y = Bv.y * ph.inv_n * ph.inv_n;
+        // fft_dir normalisation /NY/NX (src/fft_dir.f90:19), exact for N = 2^k
+        const double2 Av = make_double2(A[r].x * ph.inv_n2, A[r].y * ph.inv_n2);
+        const double2 Bv = make_double2(z[r].x * ph.inv_n2, z[r].y * ph.inv_n2);
         // src/nlt.f90:57-58
-        double2 k = make_double2(fkx * Av.y + fky * Bv.y,
-                                 -fkx * Av.x - fky * Bv.x);
-        if (ph.truncate == 1 && cut_mode(ph, fkxs, fkys))
-          k = make_double2(0.0, 0.0);
-        const size_t q = cbase + r;
+        double2 k = make_double2(fkx * Av.y + fky * Bv.y, -fkx * Av.x - fky * Bv.x);
+        if (ph.truncate == 1 && cut_mode(ph, fkxs, fkys)) k = make_double2(0.0, 0.0);
+        const unsigned q = cbase + (unsigned)row;
         const bool mean = (c == 0 && p == 0);
         if constexpr (MODE == C_NLT) {
           if (mean) k = make_double2(0.0, 0.0);  // trunc.f90:24
@@ -134,9 +155,8 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
           const double E = a.emk[q];
           const double2 zz = a.z[q];
           const double EE = E * E;
-          a.znew[q] = make_double2(EE * (zz.x + ph.dt6 * k.x),
-                                   EE * (zz.y + ph.dt6 * k.y));
-          s[m] = make_double2(E * (zz.x + ph.dt2 * k.x), E * (zz.y + ph.dt2 * k.y));
+          a.znew[q] = make_double2(EE * (zz.x + ph.dt6 * k.x), EE * (zz.y + ph.dt6 * k.y));
+          s[r] = make_double2(E * (zz.x + ph.dt2 * k.x), E * (zz.y + ph.dt2 * k.y));
         } else if constexpr (MODE == C_RK4_2) {
           // rkstep.f90:105,110
           const double E = a.emk[q];
@@ -144,7 +164,7 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
           const double2 zn = a.znew[q];
           const double d3E = ph.dt3 * E;
           a.znew[q] = make_double2(zn.x + d3E * k.x, zn.y + d3E * k.y);
-          s[m] = make_double2(E * zz.x + ph.dt2 * k.x, E * zz.y + ph.dt2 * k.y);
+          s[r] = make_double2(E * zz.x + ph.dt2 * k.x, E * zz.y + ph.dt2 * k.y);
         } else if constexpr (MODE == C_RK4_3) {
           // rkstep.f90:114,119
           const double E = a.emk[q];
@@ -152,8 +172,7 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
           const double2 zn = a.znew[q];
           const double d3E = ph.dt3 * E;
           a.znew[q] = make_double2(zn.x + d3E * k.x, zn.y + d3E * k.y);
-          s[m] = make_double2(E * (E * zz.x + ph.dt * k.x),
-                              E * (E * zz.y + ph.dt * k.y));
+          s[r] = make_double2(E * (E * zz.x + ph.dt * k.x), E * (E * zz.y + ph.dt * k.y));
         } else if constexpr (MODE == C_RK4_4) {
           // rkstep.f90:123,127
           const double2 zn = a.znew[q];
@@ -164,66 +183,70 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
           // rkstep.f90:24/34 and :42
           const double E = a.emk[q];
           const double2 zz = a.z[q];
-          s[m] = make_double2(E * (zz.x + ph.dt2 * k.x), E * (zz.y + ph.dt2 * k.y));
-          if (mean) s[m] = make_double2(0.0, 0.0);
+          s[r] = make_double2(E * (zz.x + ph.dt2 * k.x), E * (zz.y + ph.dt2 * k.y));
+          if (mean) s[r] = make_double2(0.0, 0.0);
         } else if constexpr (MODE == C_RK2_2) {
           // rkstep.f90:54/64 and :72
           const double E = a.emk[q];
           const double2 zz = a.z[q];
-          double2 zo = make_double2(E * (E * zz.x + ph.dt * k.x),
-                                    E * (E * zz.y + ph.dt * k.y));
+          double2 zo = make_double2(E * (E * zz.x + ph.dt * k.x), E * (E * zz.y + ph.dt * k.y));
           if (mean) zo = make_double2(0.0, 0.0);
           a.z[q] = zo;
         }
       }
     }
-    __syncthreads();
   } else {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int p = j + TPL * m;
-      const int r = row_of(p, rc, N);
-      s[m] = (colok && r >= 0) ? a.z[cbase + r] : make_double2(0.0, 0.0);
+    for (int r = 0; r < kE; ++r) {
+      const int p = j + TPL * r;
+      const int row = row_of(p, rc, N);
+      s[r] = (live_r<TR>(r) && colok && row >= 0) ? a.z[cbase + (unsigned)row]
+                                                   : make_double2(0.0, 0.0);
     }
   }
 
   if constexpr (INV) {
-    double ka[16];
+    // ka is kept only for the general alpha; for alpha = 2 it is k2 and is
+    // recomputed where needed.
+    double ka[AK == 1 ? 1 : kE];
+    if constexpr (AK != 1) {
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-      const int p = j + TPL * m;
-      const double fky = (double)kyi_of(p, N) * ph.dky;
-      const double k2 = fkxs + fky * fky;
-      const bool use = colok && row_of(p, rc, N) >= 0 && !(c == 0 && p == 0);
-      ka[m] = use ? kalpha(ph, k2) : 1.0;
+      for (int r = 0; r < kE; ++r) {
+        const int p = j + TPL * r;
+        const double fky = (double)kyi_of(p, N) * ph.dky;
+        const bool use = live_r<TR>(r) && colok && row_of(p, rc, N) >= 0 && !(c == 0 && p == 0);
+        ka[r] = use ? kalpha_t<AK>(ph, fkxs + fky * fky) : 1.0;
+      }
     }
 #pragma unroll 1
     for (int f = 0; f < 3; ++f) {
+      int jo = j;
+      asm volatile("" : "+v"(jo));  // keep the per-row factors inside the loop
 #pragma unroll
-      for (int m = 0; m < 16; ++m) {
-        const int p = j + TPL * m;
-        const int r = row_of(p, rc, N);
+      for (int r = 0; r < kE; ++r) {
+        const int p = jo + TPL * r;
         double2 v = make_double2(0.0, 0.0);
-        if (colok && r >= 0) {
+        if (live_r<TR>(r) && colok && row_of(p, rc, N) >= 0) {
           if (f == 0) {
-            v = s[m];
+            v = s[r];
           } else if (!(c == 0 && p == 0)) {  // nlt.f90:25
+            const double fky = (double)kyi_of(p, N) * ph.dky;
+            const double kar = AK == 1 ? fkxs + fky * fky : ka[AK == 1 ? 0 : r];
             if (f == 1) {
-              const double kyn = ((double)kyi_of(p, N) * ph.dky) / ka[m];
-              v = make_double2(-kyn * s[m].y, kyn * s[m].x);  // nlt.f90:31-32
+              const double kyn = fky / kar;
+              v = make_double2(-kyn * s[r].y, kyn * s[r].x);  // nlt.f90:31-32
             } else {
-              const double kxn = fkx / ka[m];
-              v = make_double2(kxn * s[m].y, -kxn * s[m].x);  // nlt.f90:33-34
+              const double kxn = fkx / kar;
+              v = make_double2(kxn * s[r].y, -kxn * s[r].x);  // nlt.f90:33-34
             }
           }
         }
-        buf[lpad(p)] = v;
+        z[r] = v;
       }
-      __syncthreads();
-      fft_line<LOGN, +1>(buf, j, a.tw);
-      store_tile<LOGN, BT>(a.y1 + (size_t)(f * a.ntiles + tile) * N * BT, lds,
-                           tid);
-      __syncthreads();
+      fft_reg<LOGN, +1>(z, buf, j, a.tw);
+      double2* dst = a.y1 + (size_t)(f * a.ntiles + tile) * N * BT;
+#pragma unroll
+      for (int r = 0; r < kE; ++r) dst[(unsigned)((j + TPL * r) * BT + b)] = z[r];
     }
   }
 }
@@ -236,114 +259,101 @@ template <int LOGN>
 struct RowAddr {
   static constexpr int N = 1 << LOGN, BT = bt_of(LOGN);
   int ntiles;
-  __device__ __forceinline__ size_t operator()(int f, int k, int y) const {
-    return ((size_t)(f * ntiles + k / BT) * N + y) * BT + (k % BT);
+  // element offset; below 2^32 for every supported size
+  __device__ __forceinline__ unsigned operator()(int f, int k, int y) const {
+    return ((unsigned)(f * ntiles + k / BT) * N + y) * BT + (k % BT);
   }
 };
 
-// Z = X0 + i X1 over the full x-spectrum, from the half-spectra of two real
-// rows (kx <= kx_max; the imaginary parts at kx = 0 and N/2 are not used).
+// Z = F + i G over the full x-spectrum at the thread's positions
+// p = j + TPL*r, from the half-spectra F, G of two real sequences (kx <=
+// kx_max; the imaginary parts at kx = 0 and N/2 are not used, as in a C2R
+// transform).  The inverse transform of Z is f + i g.
+// Offset of column k of a y-row in the tiled layout [tile][y][k % BT].
 template <int LOGN>
-__device__ __forceinline__ void build_pair(double2* buf, int j,
-                                           const double2* __restrict__ y,
-                                           const RowAddr<LOGN>& ad, int f,
-                                           int y0, int kxm) {
-  constexpr int N = 1 << LOGN, TPL = N / 16;
+__device__ __forceinline__ unsigned koff(int k) {
+  constexpr int N = 1 << LOGN, BT = bt_of(LOGN);
+  return (unsigned)(k / BT) * (unsigned)(N * BT) + (unsigned)(k % BT);
+}
+
+template <int LOGN, int TR>
+__device__ __forceinline__ void build_two(double2* z, int j,
+                                          const double2* __restrict__ y,
+                                          unsigned off_f, unsigned off_g, int kxm) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const int p = j + TPL * m;
+  for (int r = 0; r < kE; ++r) {
+    const int p = j + TPL * r;
     double2 v = make_double2(0.0, 0.0);
-    if (p == 0 || (p == N / 2 && kxm == N / 2)) {
-      const double2 x0 = y[ad(f, p, y0)], x1 = y[ad(f, p, y0 + 1)];
-      v = make_double2(x0.x, x1.x);
-    } else if (p < N / 2) {
-      if (p <= kxm) {
-        const double2 x0 = y[ad(f, p, y0)], x1 = y[ad(f, p, y0 + 1)];
-        v = make_double2(x0.x - x1.y, x0.y + x1.x);
-      }
-    } else if (p > N / 2) {
-      const int k = N - p;
-      if (k <= kxm) {
-        const double2 x0 = y[ad(f, k, y0)], x1 = y[ad(f, k, y0 + 1)];
-        v = make_double2(x0.x + x1.y, x1.x - x0.y);
-      }
+    const int k = p <= N / 2 ? p : N - p;
+    if (live_r<TR>(r) && k <= kxm) {
+      const double2 x0 = y[off_f + koff<LOGN>(k)];
+      const double2 x1 = y[off_g + koff<LOGN>(k)];
+      if (p == 0 || p == N / 2) v = make_double2(x0.x, x1.x);
+      else if (p < N / 2) v = make_double2(x0.x - x1.y, x0.y + x1.x);
+      else v = make_double2(x0.x + x1.y, x1.x - x0.y);
     }
-    buf[lpad(p)] = v;
+    z[r] = v;
   }
 }
-
-// Split W = FFT(a0 + i a1) into the half-spectra of the real rows a0, a1 and
-// store kx <= kx_max.
-template <int LOGN>
-__device__ __forceinline__ void split_store(const double2* buf, int j,
+// Split W = FFT(a + i b) (held at k = j + TPL*r) into the half-spectra of
+// the real sequences a and b and store kx <= kx_max.  W[N-k] comes through
+// LDS.
+template <int LOGN, int TR>
+__device__ __forceinline__ void split_store(const double2* w, double2* buf, int j,
                                             double2* __restrict__ y,
-                                            const RowAddr<LOGN>& ad, int f,
-                                            int y0, int kxm) {
-  constexpr int N = 1 << LOGN, TPL = N / 16;
+                                            unsigned off_a, unsigned off_b, int kxm) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
 #pragma unroll
-  for (int m = 0; m < 9; ++m) {
-    const int k = j + TPL * m;
-    if (k <= kxm && (m < 8 || j == 0)) {
-      const double2 W = buf[lpad(k)];
+  for (int r = 0; r < kE; ++r)
+    if (live_r<TR>(r)) buf[lpad(j + TPL * r)] = w[r];
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r <= kE / 2; ++r) {  // k <= N/2 (r = kE/2 only for k = N/2)
+    const int k = j + TPL * r;
+    if (live_r<TR>(r) && k <= kxm && k <= N / 2) {
+      const double2 W = w[r];
       const double2 Wc = buf[lpad((N - k) & (N - 1))];
-      y[ad(f, k, y0)] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
-      y[ad(f, k, y0 + 1)] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
+      y[off_a + koff<LOGN>(k)] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
+      y[off_b + koff<LOGN>(k)] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
     }
   }
+  __syncthreads();
 }
 
-template <int LOGN>
-__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
+// One y row per line: q+iv and q+iu come out of one complex transform each,
+// so each product is formed the moment its factors exist and only q*v is
+// carried (as 8 doubles) into the R2C transform of W = q*u + i q*v.
+template <int LOGN, int TR>
+__global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
     r_pass(PassArgs a) {
-  constexpr int N = 1 << LOGN, TPL = N / 16, RP = rp_of(LOGN);
+  constexpr int N = 1 << LOGN, TPL = N / kE, RP = rows_of(LOGN), BT = bt_of(LOGN);
   constexpr int LINE = Line<LOGN>::PADDED;
   __shared__ double2 lds[RP * LINE];
-  const int pl = threadIdx.x / TPL, j = threadIdx.x % TPL;
+  // Two rows over two-column tiles: lanes 4m..4m+3 take (row 0, k), (row 0,
+  // k+1), (row 1, k), (row 1, k+1), one contiguous 64-byte run of a tile.
+  const int tid = threadIdx.x;
+  const int pl = (RP == 2 && BT == 2) ? (tid >> 1) & 1 : tid / TPL;
+  const int j = (RP == 2 && BT == 2) ? (((tid >> 2) << 1) | (tid & 1)) : tid % TPL;
   const int g = xcd_remap(blockIdx.x, gridDim.x);
-  const int y0 = 2 * (g * RP + pl);
+  const int yrow = g * RP + pl;
   double2* buf = lds + pl * LINE;
   const int kxm = a.rc.kx;
-  const RowAddr<LOGN> ad{a.ntiles};
+  const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field
+  const unsigned offQ = (unsigned)yrow * BT, offU = offQ + fstride, offV = offU + fstride;
 
-  double q0[16], q1[16], a0[16], a1[16];
-  build_pair<LOGN>(buf, j, a.y1, ad, 0, y0, kxm);
-  __syncthreads();
-  fft_line<LOGN, +1>(buf, j, a.tw);
+  double2 z[kE];
+  double qv[kE];
+  build_two<LOGN, TR>(z, j, a.y1, offQ, offV, kxm);
+  fft_reg<LOGN, +1>(z, buf, j, a.tw);
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const double2 v = buf[lpad(j + TPL * m)];
-    q0[m] = v.x;
-    q1[m] = v.y;
-  }
-  __syncthreads();
-  build_pair<LOGN>(buf, j, a.y1, ad, 1, y0, kxm);
-  __syncthreads();
-  fft_line<LOGN, +1>(buf, j, a.tw);
+  for (int r = 0; r < kE; ++r) qv[r] = z[r].x * z[r].y;  // produ.f90:16
+  build_two<LOGN, TR>(z, j, a.y1, offQ, offU, kxm);
+  fft_reg<LOGN, +1>(z, buf, j, a.tw);
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const double2 v = buf[lpad(j + TPL * m)];
-    a0[m] = q0[m] * v.x;  // produ.f90:15
-    a1[m] = q1[m] * v.y;
-  }
-  __syncthreads();
-  build_pair<LOGN>(buf, j, a.y1, ad, 2, y0, kxm);
-  __syncthreads();
-  fft_line<LOGN, +1>(buf, j, a.tw);
-#pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const int x = lpad(j + TPL * m);
-    const double2 v = buf[x];
-    buf[x] = make_double2(q0[m] * v.x, q1[m] * v.y);  // produ.f90:16
-  }
-  __syncthreads();
-  fft_line<LOGN, -1>(buf, j, a.tw);
-  split_store<LOGN>(buf, j, a.y2, ad, 1, y0, kxm);
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < 16; ++m) buf[lpad(j + TPL * m)] = make_double2(a0[m], a1[m]);
-  __syncthreads();
-  fft_line<LOGN, -1>(buf, j, a.tw);
-  split_store<LOGN>(buf, j, a.y2, ad, 0, y0, kxm);
+  for (int r = 0; r < kE; ++r) z[r] = make_double2(z[r].x * z[r].y, qv[r]);  // produ.f90:15
+  fft_reg<LOGN, -1>(z, buf, j, a.tw);
+  split_store<LOGN, TR>(z, buf, j, a.y2, offQ, offQ + fstride, kxm);
 }
 
 // ---------------------------------------------------------------------------
@@ -352,9 +362,9 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
 // ---------------------------------------------------------------------------
 
 template <int LOGN>
-__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
+__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / kE)
     ref_rows_fwd(double* z, const double2* __restrict__ tw) {
-  constexpr int N = 1 << LOGN, TPL = N / 16, RP = rp_of(LOGN);
+  constexpr int N = 1 << LOGN, TPL = N / kE, RP = rp_of(LOGN);
   constexpr int LINE = Line<LOGN>::PADDED;
   __shared__ double2 lds[RP * LINE];
   const int pl = threadIdx.x / TPL, j = threadIdx.x % TPL;
@@ -363,20 +373,23 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
   const size_t ld = (size_t)N + 2;
   const double* r0 = z + ld * y0;
   const double* r1 = r0 + ld;
+  double2 w[kE];
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const int x = j + TPL * m;
-    buf[lpad(x)] = make_double2(r0[x], r1[x]);
+  for (int r = 0; r < kE; ++r) {
+    const int x = j + TPL * r;
+    w[r] = make_double2(r0[x], r1[x]);
   }
-  __syncthreads();
-  fft_line<LOGN, -1>(buf, j, tw);
+  fft_reg<LOGN, -1>(w, buf, j, tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) buf[lpad(j + TPL * r)] = w[r];
+  __syncthreads();  // also orders every read of the real rows before the writes
   double2* c0 = reinterpret_cast<double2*>(z + ld * y0);
   double2* c1 = c0 + (N / 2 + 1);
 #pragma unroll
-  for (int m = 0; m < 9; ++m) {
-    const int k = j + TPL * m;
-    if (k <= N / 2 && (m < 8 || j == 0)) {
-      const double2 W = buf[lpad(k)];
+  for (int r = 0; r < kE; ++r) {
+    const int k = j + TPL * r;
+    if (k <= N / 2) {
+      const double2 W = w[r];
       const double2 Wc = buf[lpad((N - k) & (N - 1))];
       c0[k] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
       c1[k] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
@@ -385,9 +398,9 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
 }
 
 template <int LOGN>
-__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
+__global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / kE)
     ref_rows_inv(double* z, const double2* __restrict__ tw) {
-  constexpr int N = 1 << LOGN, TPL = N / 16, RP = rp_of(LOGN);
+  constexpr int N = 1 << LOGN, TPL = N / kE, RP = rp_of(LOGN);
   constexpr int LINE = Line<LOGN>::PADDED;
   __shared__ double2 lds[RP * LINE];
   const int pl = threadIdx.x / TPL, j = threadIdx.x % TPL;
@@ -396,9 +409,10 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
   const size_t ld = (size_t)N + 2;
   const double2* c0 = reinterpret_cast<const double2*>(z + ld * y0);
   const double2* c1 = c0 + (N / 2 + 1);
+  double2 w[kE];
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const int p = j + TPL * m;
+  for (int r = 0; r < kE; ++r) {
+    const int p = j + TPL * r;
     double2 v;
     if (p == 0 || p == N / 2) {
       v = make_double2(c0[p].x, c1[p].x);
@@ -409,18 +423,17 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
       const double2 x0 = c0[N - p], x1 = c1[N - p];
       v = make_double2(x0.x + x1.y, x1.x - x0.y);
     }
-    buf[lpad(p)] = v;
+    w[r] = v;
   }
-  __syncthreads();
-  fft_line<LOGN, +1>(buf, j, tw);
+  __syncthreads();  // every read of the complex rows before any write
+  fft_reg<LOGN, +1>(w, buf, j, tw);
   double* r0 = z + ld * y0;
   double* r1 = r0 + ld;
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
-    const int x = j + TPL * m;
-    const double2 v = buf[lpad(x)];
-    r0[x] = v.x;
-    r1[x] = v.y;
+  for (int r = 0; r < kE; ++r) {
+    const int x = j + TPL * r;
+    r0[x] = w[r].x;
+    r1[x] = w[r].y;
   }
   if (j < 2) {
     r0[N + j] = 0.0;
@@ -430,32 +443,29 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / 16)
 
 // y transform of every kx column in place; SCALE applies /NY/NX.
 template <int LOGN, int S, int SCALE>
-__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / 16)
+__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE)
     ref_cols(double* z, const double2* __restrict__ tw) {
-  constexpr int N = 1 << LOGN, TPL = N / 16, BT = bt_of(LOGN);
-  constexpr int LINE = Line<LOGN>::PADDED, NTHR = BT * TPL, NH = N / 2 + 1;
+  constexpr int N = 1 << LOGN, TPL = N / kE, BT = bt_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED, NH = N / 2 + 1;
   __shared__ double2 lds[BT * LINE];
-  const int tid = threadIdx.x, b = tid / TPL, j = tid % TPL;
-  const int kx0 = blockIdx.x * BT;
+  const int tid = threadIdx.x, b = tid % BT, j = tid / BT;
+  const int kx = blockIdx.x * BT + b;
+  const bool ok = kx < NH;
   double2* zc = reinterpret_cast<double2*>(z);
+  double2 w[kE];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int e = tid + NTHR * i, y = e / BT, bb = e % BT, kx = kx0 + bb;
-    lds[bb * LINE + lpad(y)] =
-        kx < NH ? zc[kx + (size_t)NH * y] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  fft_line<LOGN, S>(lds + b * LINE, j, tw);
+  for (int r = 0; r < kE; ++r)
+    w[r] = ok ? zc[kx + (size_t)NH * (j + TPL * r)] : make_double2(0.0, 0.0);
+  fft_reg<LOGN, S>(w, lds + b * LINE, j, tw);
+  if (ok) {
 #pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int e = tid + NTHR * i, y = e / BT, bb = e % BT, kx = kx0 + bb;
-    if (kx < NH) {
-      double2 v = lds[bb * LINE + lpad(y)];
+    for (int r = 0; r < kE; ++r) {
+      double2 v = w[r];
       if (SCALE) {
         v.x = v.x / (double)N / (double)N;
         v.y = v.y / (double)N / (double)N;
       }
-      zc[kx + (size_t)NH * y] = v;
+      zc[kx + (size_t)NH * (j + TPL * r)] = v;
     }
   }
 }
@@ -474,37 +484,54 @@ template <int L>
 hipError_t ref_fft_launch(int dir, double* z, const double2* tw, hipStream_t s);
 
 #if defined(GT_INST_LOGN)
-template <int L>
-hipError_t c_pass_launch(int mode, const PassArgs& a, hipStream_t s) {
-  constexpr int T = bt_of(L) * (1 << L) / 16;
+template <int L, int AK, int TR>
+static hipError_t c_pass_launch_ak(int mode, const PassArgs& a, hipStream_t s) {
+  constexpr int T = bt_of(L) * (1 << L) / kE;
   dim3 g(a.ntiles), b(T);
   switch (mode) {
-    case C_START: c_pass<L, C_START><<<g, b, 0, s>>>(a); break;
-    case C_NLT: c_pass<L, C_NLT><<<g, b, 0, s>>>(a); break;
-    case C_RK4_1: c_pass<L, C_RK4_1><<<g, b, 0, s>>>(a); break;
-    case C_RK4_2: c_pass<L, C_RK4_2><<<g, b, 0, s>>>(a); break;
-    case C_RK4_3: c_pass<L, C_RK4_3><<<g, b, 0, s>>>(a); break;
-    case C_RK4_4: c_pass<L, C_RK4_4><<<g, b, 0, s>>>(a); break;
-    case C_RK2_1: c_pass<L, C_RK2_1><<<g, b, 0, s>>>(a); break;
-    case C_RK2_2: c_pass<L, C_RK2_2><<<g, b, 0, s>>>(a); break;
+    case C_START: c_pass<L, C_START, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_NLT: c_pass<L, C_NLT, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK4_1: c_pass<L, C_RK4_1, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK4_2: c_pass<L, C_RK4_2, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK4_3: c_pass<L, C_RK4_3, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK4_4: c_pass<L, C_RK4_4, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK2_1: c_pass<L, C_RK2_1, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK2_2: c_pass<L, C_RK2_2, AK, TR><<<g, b, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
+// TR = 1 when the rectangle lies below 3N/8 on both axes (always so with
+// the 2/3 rule); the position set of a thread is j + r*N/8.
+__host__ __device__ inline int rect_tr(const Rect& rc, int n) {
+  const int lim = 3 * (n / kE);
+  return (rc.kx < lim && rc.p < lim && rc.q < lim && n >= 16) ? 1 : 0;
+}
+
+template <int L>
+hipError_t c_pass_launch(int mode, const PassArgs& a, hipStream_t s) {
+  const int tr = rect_tr(a.rc, 1 << L);
+  if (a.ph.ahalf_is_one)
+    return tr ? c_pass_launch_ak<L, 1, 1>(mode, a, s) : c_pass_launch_ak<L, 1, 0>(mode, a, s);
+  return tr ? c_pass_launch_ak<L, 0, 1>(mode, a, s) : c_pass_launch_ak<L, 0, 0>(mode, a, s);
+}
+
 template <int L>
 hipError_t r_pass_launch(const PassArgs& a, hipStream_t s) {
-  constexpr int RP = rp_of(L), T = RP * (1 << L) / 16;
-  const int npairs = (1 << L) / 2;
-  r_pass<L><<<dim3(npairs / RP), dim3(T), 0, s>>>(a);
+  constexpr int RP = rows_of(L), T = RP * (1 << L) / kE;
+  if (rect_tr(a.rc, 1 << L))
+    r_pass<L, 1><<<dim3((1 << L) / RP), dim3(T), 0, s>>>(a);
+  else
+    r_pass<L, 0><<<dim3((1 << L) / RP), dim3(T), 0, s>>>(a);
   return hipGetLastError();
 }
 
 template <int L>
 hipError_t ref_fft_launch(int dir, double* z, const double2* tw, hipStream_t s) {
   constexpr int N = 1 << L, RP = rp_of(L), BT = bt_of(L);
-  const dim3 gr(N / 2 / RP), br(RP * N / 16);
-  const dim3 gc((N / 2 + 1 + BT - 1) / BT), bc(BT * N / 16);
+  const dim3 gr(N / 2 / RP), br(RP * N / kE);
+  const dim3 gc((N / 2 + 1 + BT - 1) / BT), bc(BT * N / kE);
   if (dir < 0) {  // fft_dir: rows R2C, columns forward, /NY/NX
     ref_rows_fwd<L><<<gr, br, 0, s>>>(z, tw);
     ref_cols<L, -1, 1><<<gc, bc, 0, s>>>(z, tw);

@@ -424,7 +424,9 @@ int r_pass(gt_ctx* c, const gt::PassArgs& a) {
 }
 
 // Forcing phases of one call: nf draws of rann (src/forcing.f90:40-51),
-// te = pi2*rann with pi2 = 2*3.14159265358979 (src/forcing.f90:10).
+// te = pi2*rann with pi2 = 2*3.14159265358979 (src/forcing.f90:10), and
+// (cos te, sin te) evaluated in extended precision and rounded once: the
+// correctly rounded values, identical whichever compiler built the caller.
 void forcing_phases(int nf, int32_t* seed, double2* out) {
   const double pi2 = 2.0 * 3.14159265358979;
   uint32_t s = (uint32_t)*seed;
@@ -432,7 +434,7 @@ void forcing_phases(int nf, int32_t* seed, double2* out) {
     s = (69069u * s + 1u) & 0x7FFFFFFFu;
     const float r = (float)((double)s / (double)0x7FFFFFFF);
     const double te = pi2 * (double)r;
-    out[k] = make_double2(std::cos(te), std::sin(te));
+    out[k] = make_double2((double)cosl((long double)te), (double)sinl((long double)te));
   }
   *seed = (int32_t)s;
 }
@@ -553,6 +555,7 @@ int gt_create(int32_t n, const double* p, gt_ctx** out) {
   ph.fkysmax = c->fkysmax;
   ph.truncate = c->truncate;
   ph.inv_n = 1.0 / (double)n;
+  ph.inv_n2 = 1.0 / ((double)n * (double)n);
   ph.nu = c->nu;
   ph.mu = c->mu;
   ph.alpnu = c->alpnu;

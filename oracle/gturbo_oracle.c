@@ -544,12 +544,17 @@ float or_rann(int32_t *seed) {
   return (float)((double)s / (double)0x7FFFFFFF);
 }
 
-/* te = pi2*rann with the truncated literal of src/forcing.f90:10. */
+/* te = pi2*rann with the truncated literal of src/forcing.f90:10.  cos and
+ * sin are taken in extended precision and rounded once (the correctly
+ * rounded values), which makes the phase table independent of how the
+ * double-precision libm is reached: the reference evaluates them with the
+ * device cos/sin of its compiler, so its own phases may differ from these
+ * by one unit in the last place. */
 static void phase(int32_t *seed, double *cc, double *cs) {
   const double pi2 = 2.0 * 3.14159265358979;
   double te = pi2 * (double)or_rann(seed);
-  *cc = cos(te);
-  *cs = sin(te);
+  *cc = (double)cosl((long double)te);
+  *cs = (double)sinl((long double)te);
 }
 
 void or_forcing_phases(const or_ctx *c, int32_t *seed, int nsteps,
