@@ -65,7 +65,7 @@ def read_traffic(kernel):
     return vals.get(kernel)
 
 
-def cpu_baseline(p, w0, seed, budget_s=12.0, max_steps=4):
+def cpu_baseline(p, w0, seed, budget_s=10.0, max_steps=40):
     import oracle
     oracle.build()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

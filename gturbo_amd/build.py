@@ -96,9 +96,13 @@ def build_fortran(verbose=False):
         return None
     mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
-    _run([fc, "-O2", f"-module-dir={mod}", f"-I{mod}", *paths, "-o", FEXE,
-          f"-L{HERE}", "-lgturbo_hip", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/.."],
-         verbose)
+    objs = []
+    for src in paths:  # in order: modules before their users
+        obj = os.path.join(mod, os.path.basename(src) + ".o")
+        _run([fc, "-O2", "-c", "-module-dir", mod, f"-I{mod}", src, "-o", obj], verbose)
+        objs.append(obj)
+    _run([fc, "-O2", *objs, "-o", FEXE, f"-L{HERE}", "-lgturbo_hip",
+          "-Wl,-rpath,$ORIGIN/.."], verbose)
     return FEXE
 
 
