@@ -123,9 +123,18 @@ def main():
     for a in (znl, znew, u, v):
         s.enter_data(a, copyin=False)
     if args.calib_copy:
-        # (N+2)*N*8 bytes read and written by the runtime's copy kernel
-        s.copy(znl, u)
+        # A plain copy of known size for scaling the HBM counters: (N+2)*N*8
+        # bytes read and written.  The source is written first and three
+        # more fields after it, so it is no longer held by the 256 MB
+        # Infinity Cache when it is copied.
+        cal = [s.field() for _ in range(4)]
+        for a in cal:
+            s.enter_data(a, copyin=False)
         s.synchronize()
+        s.copy(cal[1], cal[0])
+        s.synchronize()
+        for a in cal:
+            s.exit_data(a, copyout=False)
     s.fft_dir(z)
     s.trunc(z)
     seed = w.forcing_seed
