@@ -87,6 +87,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
   return (bid % 8) * (nblk / 8) + bid / 8;
 }
 
+// Thread -> (line, index in line) for NL lines per workgroup.  Lines
+// alternate every 16 lanes: one 16-lane LDS access group then touches a
+// single line (contiguous indices: no bank conflicts between lines), while a
+// wave's global access still covers all NL lines of a tile row or run.
+template <int TPL, int NL>
+__device__ __forceinline__ void line_map(int tid, int& line, int& j) {
+  constexpr int G = TPL < 16 ? TPL : 16;
+  const int c = tid / G;
+  line = c % NL;
+  j = (c / NL) * G + tid % G;
+}
+
 // ---------------------------------------------------------------------------
 // y pass
 // ---------------------------------------------------------------------------
@@ -100,9 +112,9 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
   constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
                        MODE == C_RK4_3 || MODE == C_RK2_1;
   __shared__ double2 lds[BT * LINE];
-  // Columns interleave over the lanes so that a tile row (BT columns at one
-  // y) is read or written by adjacent lanes.
-  const int tid = threadIdx.x, b = tid % BT, j = tid / BT;
+  const int tid = threadIdx.x;
+  int b, j;
+  line_map<TPL, BT>(tid, b, j);
   const int tile = blockIdx.x;
   const int c = tile * BT + b;
   const Rect rc = a.rc;
@@ -330,11 +342,11 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
   constexpr int N = 1 << LOGN, TPL = N / kE, RP = rows_of(LOGN), BT = bt_of(LOGN);
   constexpr int LINE = Line<LOGN>::PADDED;
   __shared__ double2 lds[RP * LINE];
-  // Two rows over two-column tiles: lanes 4m..4m+3 take (row 0, k), (row 0,
-  // k+1), (row 1, k), (row 1, k+1), one contiguous 64-byte run of a tile.
+  // Rows alternate every 16 lanes: with two rows over two-column tiles a
+  // wave's access covers whole 64-byte runs (2 rows x 2 columns) of a tile.
   const int tid = threadIdx.x;
-  const int pl = (RP == 2 && BT == 2) ? (tid >> 1) & 1 : tid / TPL;
-  const int j = (RP == 2 && BT == 2) ? (((tid >> 2) << 1) | (tid & 1)) : tid % TPL;
+  int pl, j;
+  line_map<TPL, RP>(tid, pl, j);
   const int g = xcd_remap(blockIdx.x, gridDim.x);
   const int yrow = g * RP + pl;
   double2* buf = lds + pl * LINE;
@@ -354,6 +366,44 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
   for (int r = 0; r < kE; ++r) z[r] = make_double2(z[r].x * z[r].y, qv[r]);  // produ.f90:15
   fft_reg<LOGN, -1>(z, buf, j, a.tw);
   split_store<LOGN, TR>(z, buf, j, a.y2, offQ, offQ + fstride, kxm);
+}
+
+// Row-pair variant: one line of N/8 threads handles rows y0 and y0+1.  The
+// complex transform of Q0 + iQ1 gives q of both rows; V0 + iV1 and U0 + iU1
+// likewise, and the products of the two rows are again packed two-for-one
+// for the forward transforms: five transforms per row pair, only q carried
+// across them.  With one line per workgroup two workgroups share a CU.
+template <int LOGN, int TR>
+__global__ void __launch_bounds__((1 << LOGN) / kE)
+    r_pass_pair(PassArgs a) {
+  constexpr int N = 1 << LOGN, TPL = N / kE, BT = bt_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED;
+  __shared__ double2 lds[LINE];
+  const int j = threadIdx.x;
+  const int g = xcd_remap(blockIdx.x, gridDim.x);
+  const int y0 = 2 * g;
+  const int kxm = a.rc.kx;
+  const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field
+  const unsigned r0 = (unsigned)y0 * BT, r1 = r0 + BT;    // rows y0, y0+1
+  const unsigned fq = 0, fu = fstride, fv = 2 * fstride;
+
+  double2 z[kE], qq[kE];
+  build_two<LOGN, TR>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
+  fft_reg<LOGN, +1>(z, lds, j, a.tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) qq[r] = z[r];               // q0 + i q1
+  build_two<LOGN, TR>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
+  fft_reg<LOGN, +1>(z, lds, j, a.tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
+  fft_reg<LOGN, -1>(z, lds, j, a.tw);                      // (q v)0 + i (q v)1
+  split_store<LOGN, TR>(z, lds, j, a.y2, fstride + r0, fstride + r1, kxm);
+  build_two<LOGN, TR>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
+  fft_reg<LOGN, +1>(z, lds, j, a.tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
+  fft_reg<LOGN, -1>(z, lds, j, a.tw);                      // (q u)0 + i (q u)1
+  split_store<LOGN, TR>(z, lds, j, a.y2, r0, r1, kxm);
 }
 
 // ---------------------------------------------------------------------------
@@ -448,7 +498,9 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE)
   constexpr int N = 1 << LOGN, TPL = N / kE, BT = bt_of(LOGN);
   constexpr int LINE = Line<LOGN>::PADDED, NH = N / 2 + 1;
   __shared__ double2 lds[BT * LINE];
-  const int tid = threadIdx.x, b = tid % BT, j = tid / BT;
+  const int tid = threadIdx.x;
+  int b, j;
+  line_map<TPL, BT>(tid, b, j);
   const int kx = blockIdx.x * BT + b;
   const bool ok = kx < NH;
   double2* zc = reinterpret_cast<double2*>(z);
@@ -519,6 +571,14 @@ hipError_t c_pass_launch(int mode, const PassArgs& a, hipStream_t s) {
 
 template <int L>
 hipError_t r_pass_launch(const PassArgs& a, hipStream_t s) {
+  if constexpr (L >= 11) {  // one row pair per workgroup, two workgroups per CU
+    constexpr int T = (1 << L) / kE;
+    if (rect_tr(a.rc, 1 << L))
+      r_pass_pair<L, 1><<<dim3((1 << L) / 2), dim3(T), 0, s>>>(a);
+    else
+      r_pass_pair<L, 0><<<dim3((1 << L) / 2), dim3(T), 0, s>>>(a);
+    return hipGetLastError();
+  }
   constexpr int RP = rows_of(L), T = RP * (1 << L) / kE;
   if (rect_tr(a.rc, 1 << L))
     r_pass<L, 1><<<dim3((1 << L) / RP), dim3(T), 0, s>>>(a);
