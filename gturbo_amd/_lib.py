@@ -9,7 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.environ.get("GTURBO_LIB") or os.path.join(HERE, "libgturbo_hip.so")
+LIB_PATH = os.path.join(HERE, "libgturbo_hip.so")
 
 # Every symbol include/gturbo.h declares, with (restype, argtypes).
 _P = C.c_void_p
@@ -47,6 +47,16 @@ SIGNATURES = {
     "gt_shell_index": (C.c_int, [_P, _P]),
     "gt_timing_enable": (C.c_int, [_P, _I32]),
     "gt_timing_read": (C.c_int, [_P, _P, _P]),
+    "gt_ens_create": (C.c_int, [_I32, _P, _P, C.POINTER(_P)]),
+    "gt_ens_destroy": (C.c_int, [_P]),
+    "gt_ens_set_field": (C.c_int, [_P, _I32, _P, _I32]),
+    "gt_ens_get_field": (C.c_int, [_P, _I32, _P, _I32]),
+    "gt_ens_set_seed": (C.c_int, [_P, _I32, _I32]),
+    "gt_ens_get_seed": (C.c_int, [_P, _I32, C.POINTER(_I32)]),
+    "gt_ens_schedule": (C.c_int, [_P, _P, _I32]),
+    "gt_ens_tune": (C.c_int, [_P, _I32, _P, C.POINTER(C.c_double)]),
+    "gt_ens_run": (C.c_int, [_P, _I32]),
+    "gt_ens_synchronize": (C.c_int, [_P]),
 }
 
 _lib = None

@@ -27,6 +27,7 @@ FEXE = os.path.join(FDIR, "gturbo_run")
 
 ARCH = "gfx950"
 LOGNS = range(4, 14)
+ENS_THREADS = (256, 512)  # workgroup sizes of the batched ensemble passes
 
 
 def hipcc():
@@ -45,7 +46,7 @@ def flang():
 
 
 def common_flags():
-    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    return [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
             f"-I{INCLUDE}", f"-I{CSRC}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result", "-Wno-unused-value"]
@@ -65,9 +66,15 @@ def build_library(verbose=False, jobs=None):
     cc = hipcc()
     base = common_flags()
     src_k = os.path.join(CSRC, "gturbo_kernels.hip")
-    units = [(os.path.join(OBJ, f"k{l}.o"), [src_k, f"-DGT_INST_LOGN={l}"]) for l in LOGNS]
+    units = []
+    for t in ENS_THREADS:
+        for sched in (0, 1):
+            units.append((os.path.join(OBJ, f"kens{t}_{sched}.o"),
+                          [src_k, f"-DGT_INST_ENS={t}", f"-DGT_ENS_SCHED={sched}"]))
     units.append((os.path.join(OBJ, "kcommon.o"), [src_k, "-DGT_INST_COMMON"]))
+    units += [(os.path.join(OBJ, f"k{l}.o"), [src_k, f"-DGT_INST_LOGN={l}"]) for l in LOGNS]
     units.append((os.path.join(OBJ, "runtime.o"), [os.path.join(CSRC, "gturbo_runtime.cpp")]))
+    units.append((os.path.join(OBJ, "ensemble.o"), [os.path.join(CSRC, "gturbo_ensemble.cpp")]))
     jobs = jobs or min(16, os.cpu_count() or 4)
 
     def one(u):

@@ -30,12 +30,16 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) {
 __device__ __forceinline__ double2 csub(double2 a, double2 b) {
   return make_double2(a.x - b.x, a.y - b.y);
 }
+// Products with explicit fused multiply-adds.  The kernels are compiled
+// without automatic contraction (-ffp-contract=off), so each product rounds
+// the same way in every kernel it is inlined into: a member of a batched
+// run and the same run alone give bit-identical fields.
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
 __device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
-  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return make_double2(fma(a.x, b.x, a.y * b.y), fma(a.y, b.x, -a.x * b.y));
 }
 // a * (i*S), S = +-1
 template <int S>
@@ -57,7 +61,7 @@ __device__ __forceinline__ double2 w8(double2 a) {
     constexpr double c = (m == 1 || m == 7) ? kR2 : -kR2;
     constexpr double s0 = (m == 1 || m == 3) ? kR2 : -kR2;
     constexpr double s = S > 0 ? s0 : -s0;
-    return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+    return make_double2(fma(a.x, c, -a.y * s), fma(a.x, s, a.y * c));
   }
 }
 
@@ -173,18 +177,30 @@ __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
 // Transform of one line held in registers; S = -1 forward, +1 inverse
 // (unnormalised).  Every thread of the workgroup must call it (the
 // transposes hold barriers); j is the thread's index within its line.
-template <int LOGN, int S>
+// SCHED picks the radix order when log2 N is not a multiple of 3: 0 runs
+// the radix-8 passes first and the radix-2/4 pass last, 1 the other way
+// round (same result up to rounding; the faster one depends on N).
+template <int LOGN, int S, int SCHED = 0>
 __device__ __forceinline__ void fft_reg(double2* a, double2* buf, int j,
                                         const double2* __restrict__ tw) {
   using L = Line<LOGN>;
   constexpr int NP = L::NPASS8 + (L::RLAST > 1 ? 1 : 0);
-  if constexpr (L::NPASS8 >= 1) reg_pass<LOGN, S, 8, 1, NP == 1>(a, buf, j, tw);
-  if constexpr (L::NPASS8 >= 2) reg_pass<LOGN, S, 8, 8, NP == 2>(a, buf, j, tw);
-  if constexpr (L::NPASS8 >= 3) reg_pass<LOGN, S, 8, 64, NP == 3>(a, buf, j, tw);
-  if constexpr (L::NPASS8 >= 4) reg_pass<LOGN, S, 8, 512, NP == 4>(a, buf, j, tw);
-  if constexpr (L::RLAST > 1) {
-    constexpr int NS = 1 << (3 * L::NPASS8);
-    reg_pass<LOGN, S, L::RLAST, NS, true>(a, buf, j, tw);
+  if constexpr (SCHED == 0 || L::RLAST == 1) {
+    if constexpr (L::NPASS8 >= 1) reg_pass<LOGN, S, 8, 1, NP == 1>(a, buf, j, tw);
+    if constexpr (L::NPASS8 >= 2) reg_pass<LOGN, S, 8, 8, NP == 2>(a, buf, j, tw);
+    if constexpr (L::NPASS8 >= 3) reg_pass<LOGN, S, 8, 64, NP == 3>(a, buf, j, tw);
+    if constexpr (L::NPASS8 >= 4) reg_pass<LOGN, S, 8, 512, NP == 4>(a, buf, j, tw);
+    if constexpr (L::RLAST > 1) {
+      constexpr int NS = 1 << (3 * L::NPASS8);
+      reg_pass<LOGN, S, L::RLAST, NS, true>(a, buf, j, tw);
+    }
+  } else {
+    constexpr int R0 = L::RLAST;
+    reg_pass<LOGN, S, R0, 1, NP == 1>(a, buf, j, tw);
+    if constexpr (L::NPASS8 >= 1) reg_pass<LOGN, S, 8, R0, NP == 2>(a, buf, j, tw);
+    if constexpr (L::NPASS8 >= 2) reg_pass<LOGN, S, 8, R0 * 8, NP == 3>(a, buf, j, tw);
+    if constexpr (L::NPASS8 >= 3) reg_pass<LOGN, S, 8, R0 * 64, NP == 4>(a, buf, j, tw);
+    if constexpr (L::NPASS8 >= 4) reg_pass<LOGN, S, 8, R0 * 512, NP == 5>(a, buf, j, tw);
   }
 }
 

@@ -21,7 +21,9 @@ struct Phys {
   int n;
   double dkx, dky;
   double ahalf;  // alpv/2 (nlt.f90:27)
-  int ahalf_is_one;
+  // how ka = k2**(alpv/2) is evaluated: 1 -> k2 (alpv = 2), 2 -> sqrt(k2)
+  // (alpv = 1), 3 -> sqrt(sqrt(k2)) (alpv = 1/2), 0 -> pow(k2, alpv/2)
+  int akind;
   double dt, dt2, dt3, dt6;
   double rtrunc2, fkxsmax, fkysmax;
   int truncate;
@@ -43,6 +45,10 @@ struct PassArgs {
   double2* znew;      // RK4 accumulator (compact)
   const double* emk;  // integrating factor (compact, real)
   double2* out;       // nonlinear term (compact), NLT mode
+  const double* ka;   // k2**(alpv/2) (compact), batched passes with akind 0
+  // batched ensemble passes only: x-pass packing (0 = one row per line,
+  // 1 = row pairs)
+  int xvar;
 };
 
 enum CMode : int {
@@ -108,6 +114,21 @@ struct ForcedMode {
   double ff1, ff2;
 };
 
+// Forced modes of one ensemble member for the batched forcing launch.
+struct EnsForce {
+  const ForcedMode* modes;
+  const double2* phases;  // [step][nf] of the current chunk of steps
+  double2* z;
+  double amp;             // sqrt(dt)*famp
+  int nf;
+  int first;  // index of the member's first mode in the flattened launch
+};
+
+// Sizes the batched ensemble passes are compiled for (log2 N) and their
+// workgroup sizes (threads; a workgroup holds T*8/N lines of a size).
+constexpr int kEnsMinLogn = 7, kEnsMaxLogn = 11;
+__host__ __device__ constexpr int ens_lines(int t, int logn) { return t * 8 >> logn; }
+
 // Host-side launchers (gturbo_kernels.hip).
 hipError_t launch_c_pass(int logn, int mode, const PassArgs& a, hipStream_t s);
 hipError_t launch_r_pass(int logn, const PassArgs& a, hipStream_t s);
@@ -129,5 +150,18 @@ hipError_t launch_forcing(const Phys& ph, int nf, const ForcedMode* modes,
 hipError_t launch_spectrum(const DiagArgs& d, int which, hipStream_t s);
 hipError_t launch_glob(const DiagArgs& d, hipStream_t s);
 hipError_t launch_shell_index(const Phys& ph, int* ib, hipStream_t s);
+// Batched ensemble passes: items are (member << 16 | tile or row group),
+// grid workgroups loop over them; sched is the radix order (fft_reg SCHED).
+hipError_t launch_ens_c_pass(int ty, int sched, int mode, const PassArgs* descs,
+                             const uint32_t* items, int nitems, int grid,
+                             hipStream_t s);
+hipError_t launch_ens_r_pass(int tx, int ty, int sched, const PassArgs* descs,
+                             const uint32_t* items, int nitems, int grid,
+                             hipStream_t s);
+hipError_t launch_ens_forcing(const EnsForce* f, int nmem, int total,
+                              const int* step, hipStream_t s);
+hipError_t launch_ens_tick(int* step, hipStream_t s);
+hipError_t launch_ka_table(const Phys& ph, const Rect& rc, int kyp, double* ka,
+                           hipStream_t s);
 
 }  // namespace gt

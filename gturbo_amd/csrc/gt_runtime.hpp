@@ -1,0 +1,136 @@
+// gt_runtime.hpp -- host-side state shared by the runtime translation units
+// (single-run context, batched ensemble, adjoint): the context, the present
+// table entries and the helpers that act on them.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gt_internal.hpp"
+#include "gturbo.h"
+
+using gt::ForcedMode;
+using gt::Phys;
+using gt::Rect;
+
+namespace gtr {
+
+extern thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+#define HIPCHK(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(GT_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                       \
+  } while (0)
+
+#define CHK(call)            \
+  do {                       \
+    int r_ = (call);         \
+    if (r_ != GT_OK) return r_; \
+  } while (0)
+
+constexpr int kNFORC = 2048;  // src/modules.f90:41
+constexpr int kRing = 64;     // forcing-phase slots in flight
+
+struct Mirror {
+  void* host = nullptr;
+  size_t bytes = 0;
+  double* dref = nullptr;   // reference layout
+  bool owned = true;
+  double2* dcmp = nullptr;  // compact layout
+  size_t cmp_cap = 0;       // double2 capacity of dcmp
+  Rect rect{0, 0, 0};
+  int kyp = 0;
+  bool ref_valid = true;
+  bool cmp_valid = false;
+};
+
+struct RectTables {
+  Rect rc{-1, -1, -1};
+  int kyp = 0;
+  double* emk = nullptr;
+  ForcedMode* modes = nullptr;
+};
+
+struct TimedLaunch {
+  int cls;
+  hipEvent_t a, b;
+};
+
+}  // namespace gtr
+
+struct gt_ctx {
+  int n = 0, logn = 0, nh = 0;
+  // commphy / commsim / commforc / commk (src/modules.f90)
+  double xlx, xly, dkx, dky, nu, mu, alpnu, alpmu, alpv;
+  double dt, dt2, dt3, dt6, sdt, inpz = 0, inpe = 0;
+  double famp, k1f, k2f, rtrunc, fkxsmax, fkysmax;
+  int npas, iout, imix, split, rko, truncate;
+  int nf = 0;
+  std::vector<int32_t> ikxf, ikyf, jc;
+  std::vector<double> ff, fkx, fky, fkxs, fkys, emk, emkx, emky;
+
+  Phys ph{};
+  Rect rtr{0, 0, 0}, rfull{0, 0, 0};
+
+  hipStream_t stream = nullptr;
+  double2* tw = nullptr;
+  gtr::RectTables tab[2];  // [0] truncated rect, [1] full rect
+  double2* y1 = nullptr;
+  double2* y2 = nullptr;
+  size_t ycap = 0;  // double2 per field
+  int* flag = nullptr;
+
+  // forcing phases
+  double2* ph_host = nullptr;  // pinned, kRing * nf
+  double2* ph_dev = nullptr;
+  hipEvent_t ph_ev[gtr::kRing];
+  bool ph_ev_used[gtr::kRing] = {};
+  unsigned ph_next = 0;
+
+  // diagnostics
+  double* dg = nullptr;
+  size_t dg_cap = 0;
+
+  // timing
+  bool timing = false;
+  std::vector<gtr::TimedLaunch> tl;
+  std::vector<hipEvent_t> ev_pool;
+
+  std::unordered_map<const void*, gtr::Mirror> present;
+};
+
+namespace gtr {
+
+int ilog2(int n);
+int kyp_of(const Rect& r);
+int rect_kind(const gt_ctx* c, const Rect& r);
+int rect_tables(gt_ctx* c, const Rect& r, RectTables** out);
+int ensure_ybuf(gt_ctx* c, const Rect& r);
+int lookup(gt_ctx* c, const void* host, Mirror** m);
+int alloc_cmp(gt_ctx* c, Mirror& m, const Rect& r);
+int ensure_ref(gt_ctx* c, Mirror& m);
+int ensure_cmp(gt_ctx* c, Mirror& m);
+void set_cmp_only(Mirror& m);
+void set_ref_only(Mirror& m);
+gt::PassArgs pass_args(gt_ctx* c, Mirror& z, RectTables* t);
+int c_pass(gt_ctx* c, int mode, const gt::PassArgs& a);
+int r_pass(gt_ctx* c, const gt::PassArgs& a);
+void forcing_phases(int nf, int32_t* seed, double2* out);
+
+// Timer of one launch class on the context's stream (GT_K_* of gturbo.h).
+struct Timer {
+  gt_ctx* c;
+  int cls;
+  hipEvent_t a = nullptr;
+  Timer(gt_ctx* c_, int cls_);
+  ~Timer();
+};
+
+}  // namespace gtr

@@ -35,17 +35,28 @@ __device__ __forceinline__ bool cut_mode(const Phys& ph, double fkxs,
   return k2 > ph.rtrunc2;
 }
 
+// ka = k2**(alpv/2) (src/nlt.f90:27): k2 itself for alpv = 2, correctly
+// rounded square roots for alpv = 1 and 1/2, the power otherwise.
 __device__ __forceinline__ double kalpha(const Phys& ph, double k2) {
-  // ka = k2**(alpv/2) (src/nlt.f90:27)
-  return ph.ahalf_is_one ? k2 : pow(k2, ph.ahalf);
+  switch (ph.akind) {
+    case 1: return k2;
+    case 2: return sqrt(k2);
+    case 3: return sqrt(sqrt(k2));
+    default: return pow(k2, ph.ahalf);
+  }
 }
 
-// ka for a kernel specialised on the alpha kind: AK = 1 when alpv/2 = 1
-// (ka = k2 exactly, no pow code in the kernel), else the general power.
+// ka in a y pass specialised on the alpha kind: AK = 1 when alpv = 2 (ka =
+// k2, nothing else in the kernel), AK = 0 a per-thread table of ka built
+// once per column, AK = 2 ka where it is used (the batched passes, whose
+// members differ in alpha): k2 for alpv = 2, else the member's table of ka
+// in the compact layout, built on the host with the same expressions as
+// kalpha (no power or root code in those kernels).
 template <int AK>
-__device__ __forceinline__ double kalpha_t(const Phys& ph, double k2) {
+__device__ __forceinline__ double kalpha_t(const PassArgs& a, double k2, unsigned q) {
   if constexpr (AK == 1) return k2;
-  else return pow(k2, ph.ahalf);
+  else if constexpr (AK == 0) return kalpha(a.ph, k2);
+  else return a.ph.akind == 1 ? k2 : a.ka[q];
 }
 
 // With the 2/3 rule the kept rectangle stays below |k| = 3N/8, so of a
@@ -103,19 +114,18 @@ __device__ __forceinline__ void line_map(int tid, int& line, int& j) {
 // y pass
 // ---------------------------------------------------------------------------
 
-template <int LOGN, int MODE, int AK, int TR>
-__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 1)
-    c_pass(PassArgs a) {
-  constexpr int N = 1 << LOGN, TPL = N / kE, BT = bt_of(LOGN);
+// One tile of BT kx-columns; lds holds BT padded lines.  SCHED is the radix
+// order of the transforms (fft_reg).
+template <int LOGN, int MODE, int AK, int TR, int BT, int SCHED>
+__device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
+                                       double2* lds) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
   constexpr int LINE = Line<LOGN>::PADDED;
   constexpr bool FWD = MODE != C_START;
   constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
                        MODE == C_RK4_3 || MODE == C_RK2_1;
-  __shared__ double2 lds[BT * LINE];
-  const int tid = threadIdx.x;
   int b, j;
   line_map<TPL, BT>(tid, b, j);
-  const int tile = blockIdx.x;
   const int c = tile * BT + b;
   const Rect rc = a.rc;
   const bool colok = c <= rc.kx;
@@ -132,13 +142,13 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
     const double2* srcA = a.y2 + (size_t)tile * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcA[(unsigned)((j + TPL * r) * BT + b)];
-    fft_reg<LOGN, -1>(z, buf, j, a.tw);
+    fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);
 #pragma unroll
     for (int r = 0; r < kE; ++r) A[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
     const double2* srcB = a.y2 + (size_t)(a.ntiles + tile) * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcB[(unsigned)((j + TPL * r) * BT + b)];
-    fft_reg<LOGN, -1>(z, buf, j, a.tw);
+    fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);
     int je = j;
     // The epilogue's indices (and so its loads) depend on the transform's
     // output: nothing of the epilogue is scheduled inside the last pass.
@@ -218,16 +228,16 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
   }
 
   if constexpr (INV) {
-    // ka is kept only for the general alpha; for alpha = 2 it is k2 and is
-    // recomputed where needed.
-    double ka[AK == 1 ? 1 : kE];
-    if constexpr (AK != 1) {
+    // ka is tabled only for AK = 0; for AK = 1 it is k2 and for AK = 2 it is
+    // evaluated where needed.
+    double ka[AK == 0 ? kE : 1];
+    if constexpr (AK == 0) {
 #pragma unroll
       for (int r = 0; r < kE; ++r) {
         const int p = j + TPL * r;
         const double fky = (double)kyi_of(p, N) * ph.dky;
         const bool use = live_r<TR>(r) && colok && row_of(p, rc, N) >= 0 && !(c == 0 && p == 0);
-        ka[r] = use ? kalpha_t<AK>(ph, fkxs + fky * fky) : 1.0;
+        ka[r] = use ? kalpha(ph, fkxs + fky * fky) : 1.0;
       }
     }
 #pragma unroll 1
@@ -243,7 +253,9 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
             v = s[r];
           } else if (!(c == 0 && p == 0)) {  // nlt.f90:25
             const double fky = (double)kyi_of(p, N) * ph.dky;
-            const double kar = AK == 1 ? fkxs + fky * fky : ka[AK == 1 ? 0 : r];
+            const double kar =
+                AK == 0 ? ka[AK == 0 ? r : 0]
+                        : kalpha_t<AK>(a, fkxs + fky * fky, cbase + (unsigned)row_of(p, rc, N));
             if (f == 1) {
               const double kyn = fky / kar;
               v = make_double2(-kyn * s[r].y, kyn * s[r].x);  // nlt.f90:31-32
@@ -255,7 +267,7 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
         }
         z[r] = v;
       }
-      fft_reg<LOGN, +1>(z, buf, j, a.tw);
+      fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
       double2* dst = a.y1 + (size_t)(f * a.ntiles + tile) * N * BT;
 #pragma unroll
       for (int r = 0; r < kE; ++r) dst[(unsigned)((j + TPL * r) * BT + b)] = z[r];
@@ -263,32 +275,30 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
   }
 }
 
+template <int LOGN, int MODE, int AK, int TR>
+__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 1)
+    c_pass(PassArgs a) {
+  constexpr int BT = bt_of(LOGN);
+  __shared__ double2 lds[BT * Line<LOGN>::PADDED];
+  c_body<LOGN, MODE, AK, TR, BT, 0>(a, blockIdx.x, threadIdx.x, lds);
+}
+
 // ---------------------------------------------------------------------------
 // x pass
 // ---------------------------------------------------------------------------
-
-template <int LOGN>
-struct RowAddr {
-  static constexpr int N = 1 << LOGN, BT = bt_of(LOGN);
-  int ntiles;
-  // element offset; below 2^32 for every supported size
-  __device__ __forceinline__ unsigned operator()(int f, int k, int y) const {
-    return ((unsigned)(f * ntiles + k / BT) * N + y) * BT + (k % BT);
-  }
-};
 
 // Z = F + i G over the full x-spectrum at the thread's positions
 // p = j + TPL*r, from the half-spectra F, G of two real sequences (kx <=
 // kx_max; the imaginary parts at kx = 0 and N/2 are not used, as in a C2R
 // transform).  The inverse transform of Z is f + i g.
 // Offset of column k of a y-row in the tiled layout [tile][y][k % BT].
-template <int LOGN>
+template <int LOGN, int BT>
 __device__ __forceinline__ unsigned koff(int k) {
-  constexpr int N = 1 << LOGN, BT = bt_of(LOGN);
+  constexpr int N = 1 << LOGN;
   return (unsigned)(k / BT) * (unsigned)(N * BT) + (unsigned)(k % BT);
 }
 
-template <int LOGN, int TR>
+template <int LOGN, int TR, int BT>
 __device__ __forceinline__ void build_two(double2* z, int j,
                                           const double2* __restrict__ y,
                                           unsigned off_f, unsigned off_g, int kxm) {
@@ -299,8 +309,8 @@ __device__ __forceinline__ void build_two(double2* z, int j,
     double2 v = make_double2(0.0, 0.0);
     const int k = p <= N / 2 ? p : N - p;
     if (live_r<TR>(r) && k <= kxm) {
-      const double2 x0 = y[off_f + koff<LOGN>(k)];
-      const double2 x1 = y[off_g + koff<LOGN>(k)];
+      const double2 x0 = y[off_f + koff<LOGN, BT>(k)];
+      const double2 x1 = y[off_g + koff<LOGN, BT>(k)];
       if (p == 0 || p == N / 2) v = make_double2(x0.x, x1.x);
       else if (p < N / 2) v = make_double2(x0.x - x1.y, x0.y + x1.x);
       else v = make_double2(x0.x + x1.y, x1.x - x0.y);
@@ -311,7 +321,7 @@ __device__ __forceinline__ void build_two(double2* z, int j,
 // Split W = FFT(a + i b) (held at k = j + TPL*r) into the half-spectra of
 // the real sequences a and b and store kx <= kx_max.  W[N-k] comes through
 // LDS.
-template <int LOGN, int TR>
+template <int LOGN, int TR, int BT>
 __device__ __forceinline__ void split_store(const double2* w, double2* buf, int j,
                                             double2* __restrict__ y,
                                             unsigned off_a, unsigned off_b, int kxm) {
@@ -326,29 +336,27 @@ __device__ __forceinline__ void split_store(const double2* w, double2* buf, int 
     if (live_r<TR>(r) && k <= kxm && k <= N / 2) {
       const double2 W = w[r];
       const double2 Wc = buf[lpad((N - k) & (N - 1))];
-      y[off_a + koff<LOGN>(k)] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
-      y[off_b + koff<LOGN>(k)] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
+      y[off_a + koff<LOGN, BT>(k)] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
+      y[off_b + koff<LOGN, BT>(k)] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
     }
   }
   __syncthreads();
 }
 
-// One y row per line: q+iv and q+iu come out of one complex transform each,
-// so each product is formed the moment its factors exist and only q*v is
-// carried (as 8 doubles) into the R2C transform of W = q*u + i q*v.
-template <int LOGN, int TR>
-__global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
-    r_pass(PassArgs a) {
-  constexpr int N = 1 << LOGN, TPL = N / kE, RP = rows_of(LOGN), BT = bt_of(LOGN);
+// One y row per line, NL lines per workgroup (group g holds rows g*NL ..):
+// q+iv and q+iu come out of one complex transform each, so each product is
+// formed the moment its factors exist and only q*v is carried (as 8
+// doubles) into the R2C transform of W = q*u + i q*v.  BT is the tile width
+// of the y-pass buffers.  Lines alternate every 16 lanes: with two rows over
+// two-column tiles a wave's access covers whole 64-byte runs of a tile.
+template <int LOGN, int TR, int BT, int NL, int SCHED>
+__device__ __forceinline__ void r_body(const PassArgs& a, int g, int tid,
+                                       double2* lds) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
   constexpr int LINE = Line<LOGN>::PADDED;
-  __shared__ double2 lds[RP * LINE];
-  // Rows alternate every 16 lanes: with two rows over two-column tiles a
-  // wave's access covers whole 64-byte runs (2 rows x 2 columns) of a tile.
-  const int tid = threadIdx.x;
   int pl, j;
-  line_map<TPL, RP>(tid, pl, j);
-  const int g = xcd_remap(blockIdx.x, gridDim.x);
-  const int yrow = g * RP + pl;
+  line_map<TPL, NL>(tid, pl, j);
+  const int yrow = g * NL + pl;
   double2* buf = lds + pl * LINE;
   const int kxm = a.rc.kx;
   const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field
@@ -356,54 +364,71 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
 
   double2 z[kE];
   double qv[kE];
-  build_two<LOGN, TR>(z, j, a.y1, offQ, offV, kxm);
-  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+  build_two<LOGN, TR, BT>(z, j, a.y1, offQ, offV, kxm);
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) qv[r] = z[r].x * z[r].y;  // produ.f90:16
-  build_two<LOGN, TR>(z, j, a.y1, offQ, offU, kxm);
-  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+  build_two<LOGN, TR, BT>(z, j, a.y1, offQ, offU, kxm);
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(z[r].x * z[r].y, qv[r]);  // produ.f90:15
-  fft_reg<LOGN, -1>(z, buf, j, a.tw);
-  split_store<LOGN, TR>(z, buf, j, a.y2, offQ, offQ + fstride, kxm);
+  fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);
+  split_store<LOGN, TR, BT>(z, buf, j, a.y2, offQ, offQ + fstride, kxm);
 }
 
-// Row-pair variant: one line of N/8 threads handles rows y0 and y0+1.  The
-// complex transform of Q0 + iQ1 gives q of both rows; V0 + iV1 and U0 + iU1
-// likewise, and the products of the two rows are again packed two-for-one
-// for the forward transforms: five transforms per row pair, only q carried
-// across them.  With one line per workgroup two workgroups share a CU.
-template <int LOGN, int TR>
-__global__ void __launch_bounds__((1 << LOGN) / kE)
-    r_pass_pair(PassArgs a) {
-  constexpr int N = 1 << LOGN, TPL = N / kE, BT = bt_of(LOGN);
+// Row-pair variant: each line of N/8 threads handles rows y0 and y0+1
+// (group g holds pairs g*NL ..).  The complex transform of Q0 + iQ1 gives q
+// of both rows; V0 + iV1 and U0 + iU1 likewise, and the products of the two
+// rows are again packed two-for-one for the forward transforms: five
+// transforms per row pair, only q carried across them.
+template <int LOGN, int TR, int BT, int NL, int SCHED>
+__device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
+                                          double2* lds) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
   constexpr int LINE = Line<LOGN>::PADDED;
-  __shared__ double2 lds[LINE];
-  const int j = threadIdx.x;
-  const int g = xcd_remap(blockIdx.x, gridDim.x);
-  const int y0 = 2 * g;
+  int pl, j;
+  line_map<TPL, NL>(tid, pl, j);
+  double2* buf = lds + pl * LINE;
+  const int y0 = 2 * (g * NL + pl);
   const int kxm = a.rc.kx;
   const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field
   const unsigned r0 = (unsigned)y0 * BT, r1 = r0 + BT;    // rows y0, y0+1
   const unsigned fq = 0, fu = fstride, fv = 2 * fstride;
 
   double2 z[kE], qq[kE];
-  build_two<LOGN, TR>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
-  fft_reg<LOGN, +1>(z, lds, j, a.tw);
+  build_two<LOGN, TR, BT>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
 #pragma unroll
-  for (int r = 0; r < kE; ++r) qq[r] = z[r];               // q0 + i q1
-  build_two<LOGN, TR>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
-  fft_reg<LOGN, +1>(z, lds, j, a.tw);
-#pragma unroll
-  for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1>(z, lds, j, a.tw);                      // (q v)0 + i (q v)1
-  split_store<LOGN, TR>(z, lds, j, a.y2, fstride + r0, fstride + r1, kxm);
-  build_two<LOGN, TR>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
-  fft_reg<LOGN, +1>(z, lds, j, a.tw);
+  for (int r = 0; r < kE; ++r) qq[r] = z[r];                   // q0 + i q1
+  build_two<LOGN, TR, BT>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1>(z, lds, j, a.tw);                      // (q u)0 + i (q u)1
-  split_store<LOGN, TR>(z, lds, j, a.y2, r0, r1, kxm);
+  fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);                   // (q v)0 + i (q v)1
+  split_store<LOGN, TR, BT>(z, buf, j, a.y2, fstride + r0, fstride + r1, kxm);
+  build_two<LOGN, TR, BT>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
+  fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);                   // (q u)0 + i (q u)1
+  split_store<LOGN, TR, BT>(z, buf, j, a.y2, r0, r1, kxm);
+}
+
+template <int LOGN, int TR>
+__global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
+    r_pass(PassArgs a) {
+  constexpr int RP = rows_of(LOGN);
+  __shared__ double2 lds[RP * Line<LOGN>::PADDED];
+  r_body<LOGN, TR, bt_of(LOGN), RP, 0>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x, lds);
+}
+
+// One row pair per workgroup: two workgroups share a CU.
+template <int LOGN, int TR>
+__global__ void __launch_bounds__((1 << LOGN) / kE)
+    r_pass_pair(PassArgs a) {
+  __shared__ double2 lds[Line<LOGN>::PADDED];
+  pair_body<LOGN, TR, bt_of(LOGN), 1, 0>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x,
+                                         lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -534,6 +559,12 @@ template <int L>
 hipError_t r_pass_launch(const PassArgs& a, hipStream_t s);
 template <int L>
 hipError_t ref_fft_launch(int dir, double* z, const double2* tw, hipStream_t s);
+template <int T, int SCHED>
+hipError_t ens_c_launch(int mode, const PassArgs* d, const uint32_t* it, int n, int grid,
+                        hipStream_t s);
+template <int TX, int SCHED>
+hipError_t ens_r_launch(int ty, const PassArgs* d, const uint32_t* it, int n, int grid,
+                        hipStream_t s);
 
 #if defined(GT_INST_LOGN)
 template <int L, int AK, int TR>
@@ -564,7 +595,7 @@ __host__ __device__ inline int rect_tr(const Rect& rc, int n) {
 template <int L>
 hipError_t c_pass_launch(int mode, const PassArgs& a, hipStream_t s) {
   const int tr = rect_tr(a.rc, 1 << L);
-  if (a.ph.ahalf_is_one)
+  if (a.ph.akind == 1)
     return tr ? c_pass_launch_ak<L, 1, 1>(mode, a, s) : c_pass_launch_ak<L, 1, 0>(mode, a, s);
   return tr ? c_pass_launch_ak<L, 0, 1>(mode, a, s) : c_pass_launch_ak<L, 0, 0>(mode, a, s);
 }
@@ -606,6 +637,113 @@ template hipError_t c_pass_launch<GT_INST_LOGN>(int, const PassArgs&, hipStream_
 template hipError_t r_pass_launch<GT_INST_LOGN>(const PassArgs&, hipStream_t);
 template hipError_t ref_fft_launch<GT_INST_LOGN>(int, double*, const double2*, hipStream_t);
 #endif  // GT_INST_LOGN
+
+#if defined(GT_INST_ENS)
+// ---------------------------------------------------------------------------
+// Batched ensemble passes.  One launch covers the pass of every member,
+// whatever its N and alpha: the workgroups loop over a list of work items
+// (member, tile) and run the pass body of the member's size.  For T threads
+// a workgroup holds T*8/N lines of N points, i.e. T*8*17/16 double2 of LDS
+// for every N, so the sizes share one LDS allocation.  Alpha enters through
+// kalpha_t<2>, the x-pass packing per member through PassArgs::xvar, the
+// radix order (SCHED) per launch.  Every member has the 2/3 rectangle
+// (TR = 1).  Compiled once per (T, SCHED): -DGT_INST_ENS=T
+// -DGT_ENS_SCHED=SCHED.
+// ---------------------------------------------------------------------------
+
+template <int T, int MODE, int SCHED, int L>
+__device__ __forceinline__ void ens_c_one(const PassArgs& a, int tile, int tid,
+                                          double2* lds) {
+  c_body<L, MODE, 2, 1, ens_lines(T, L), SCHED>(a, tile, tid, lds);
+}
+
+template <int T, int MODE, int SCHED>
+__global__ void __launch_bounds__(T, 4)
+    ens_c_pass(const PassArgs* __restrict__ descs, const uint32_t* __restrict__ items,
+               int nitems) {
+  __shared__ double2 lds[T * 17 / 2];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const uint32_t it = items[w];
+    const PassArgs& a = descs[it >> 16];
+    const int tile = (int)(it & 0xFFFFu);
+    // Opaque per item: nothing derived from the thread index is hoisted out
+    // of the loop (and kept live over every size's code).
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    switch (a.ph.n) {
+      case 128: ens_c_one<T, MODE, SCHED, 7>(a, tile, tid, lds); break;
+      case 256: ens_c_one<T, MODE, SCHED, 8>(a, tile, tid, lds); break;
+      case 512: ens_c_one<T, MODE, SCHED, 9>(a, tile, tid, lds); break;
+      case 1024: ens_c_one<T, MODE, SCHED, 10>(a, tile, tid, lds); break;
+      case 2048: ens_c_one<T, MODE, SCHED, 11>(a, tile, tid, lds); break;
+      default: break;
+    }
+  }
+}
+
+template <int TX, int TY, int SCHED, int L>
+__device__ __forceinline__ void ens_r_one(const PassArgs& a, int g, int tid, double2* lds) {
+  constexpr int NL = ens_lines(TX, L), BT = ens_lines(TY, L);
+  if (a.xvar) pair_body<L, 1, BT, NL, SCHED>(a, g, tid, lds);
+  else r_body<L, 1, BT, NL, SCHED>(a, g, tid, lds);
+}
+
+template <int TX, int TY, int SCHED>
+__global__ void __launch_bounds__(TX, 4)
+    ens_r_pass(const PassArgs* __restrict__ descs, const uint32_t* __restrict__ items,
+               int nitems) {
+  __shared__ double2 lds[TX * 17 / 2];
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const uint32_t it = items[w];
+    const PassArgs& a = descs[it >> 16];
+    const int g = (int)(it & 0xFFFFu);
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    switch (a.ph.n) {
+      case 128: ens_r_one<TX, TY, SCHED, 7>(a, g, tid, lds); break;
+      case 256: ens_r_one<TX, TY, SCHED, 8>(a, g, tid, lds); break;
+      case 512: ens_r_one<TX, TY, SCHED, 9>(a, g, tid, lds); break;
+      case 1024: ens_r_one<TX, TY, SCHED, 10>(a, g, tid, lds); break;
+      case 2048: ens_r_one<TX, TY, SCHED, 11>(a, g, tid, lds); break;
+      default: break;
+    }
+  }
+}
+
+template <int T, int SCHED>
+hipError_t ens_c_launch(int mode, const PassArgs* d, const uint32_t* it, int n, int grid,
+                        hipStream_t s) {
+  const dim3 g(grid), b(T);
+  switch (mode) {
+    case C_START: ens_c_pass<T, C_START, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK4_1: ens_c_pass<T, C_RK4_1, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK4_2: ens_c_pass<T, C_RK4_2, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK4_3: ens_c_pass<T, C_RK4_3, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK4_4: ens_c_pass<T, C_RK4_4, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK2_1: ens_c_pass<T, C_RK2_1, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK2_2: ens_c_pass<T, C_RK2_2, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <int TX, int SCHED>
+hipError_t ens_r_launch(int ty, const PassArgs* d, const uint32_t* it, int n, int grid,
+                        hipStream_t s) {
+  const dim3 g(grid), b(TX);
+  if (ty == 256) ens_r_pass<TX, 256, SCHED><<<g, b, 0, s>>>(d, it, n);
+  else if (ty == 512) ens_r_pass<TX, 512, SCHED><<<g, b, 0, s>>>(d, it, n);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template hipError_t ens_c_launch<GT_INST_ENS, GT_ENS_SCHED>(int, const PassArgs*,
+                                                            const uint32_t*, int, int,
+                                                            hipStream_t);
+template hipError_t ens_r_launch<GT_INST_ENS, GT_ENS_SCHED>(int, const PassArgs*,
+                                                            const uint32_t*, int, int,
+                                                            hipStream_t);
+#endif  // GT_INST_ENS
 
 #if defined(GT_INST_COMMON)
 // ---------------------------------------------------------------------------
@@ -673,6 +811,22 @@ __global__ void gather_k(Phys ph, Rect rc, int kyp, const double2* __restrict__ 
   }
 }
 
+// ka = k2**(alpv/2) over the compact rectangle, with the expressions of the
+// y pass (c_body, AK = 0), so that a batched member reading the table and a
+// single run computing ka in place get the same values.
+__global__ void ka_table_k(Phys ph, Rect rc, int kyp, double* ka) {
+  const int n = ph.n, ky = rc.ky();
+  const size_t tot = (size_t)rc.kc() * ky;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < tot;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / ky), r = (int)(e % ky);
+    const double fkx = (double)c * ph.dkx;
+    const double fkxs = fkx * fkx;
+    const double fky = (double)kyi_of(fft_of(r, rc, n), n) * ph.dky;
+    ka[(size_t)c * kyp + r] = (c == 0 && r == 0) ? 1.0 : kalpha(ph, fkxs + fky * fky);
+  }
+}
+
 // Flags a nonzero reference mode outside the rectangle.
 __global__ void outside_k(Phys ph, Rect rc, const double2* __restrict__ ref,
                           int* flag) {
@@ -705,14 +859,11 @@ __global__ void scatter_k(Phys ph, Rect rc, int kyp, const double2* __restrict__
 // src/forcing.f90:16-33.  One thread per forced mode: no two modes share an
 // element, and a mode's own updates (itself, then its Hermitian partner) run
 // in the reference's order.  amp = sdt*famp.
-__global__ void forcing_k(int nf, const ForcedMode* __restrict__ fm,
-                          const double2* __restrict__ ph, double amp,
-                          double2* z) {
+// z(i,j) += sdt*famp*(ff1 cos te, ff2 sin te) and the conjugate update of
+// the Hermitian partner (src/forcing.f90:25-31).
+__device__ __forceinline__ void force_one(const ForcedMode& f, double2 cs, double amp,
+                                          double2* z) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nf) return;
-  const ForcedMode f = fm[k];
-  const double2 cs = ph[k];
   const double a1 = amp * f.ff1 * cs.x, a2 = amp * f.ff2 * cs.y;
   double2 v = z[f.idx];
   v.x = v.x + a1;
@@ -725,6 +876,34 @@ __global__ void forcing_k(int nf, const ForcedMode* __restrict__ fm,
     z[f.pidx] = w;
   }
 }
+
+__global__ void forcing_k(int nf, const ForcedMode* __restrict__ fm,
+                          const double2* __restrict__ ph, double amp,
+                          double2* z) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nf) return;
+  force_one(fm[k], ph[k], amp, z);
+}
+
+// Forcing of every ensemble member in one launch, one thread per forced
+// mode; the phases of chunk step *step are used.
+__global__ void ens_forcing_k(const EnsForce* __restrict__ f, int nmem, int total,
+                              const int* __restrict__ step) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = nmem - 1;  // last member whose first mode is <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (f[mid].first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const EnsForce& e = f[lo];
+  const int k = t - e.first;
+  if (k >= e.nf) return;
+  force_one(e.modes[k], e.phases[(size_t)(*step) * e.nf + k], e.amp, e.z);
+}
+
+__global__ void ens_tick_k(int* step) { step[0] = step[0] + 1; }
 
 // ---------------------------------------------------------------------------
 // Diagnostics (src/diagnostics.f90), deterministic: per column the modes of a
@@ -982,6 +1161,47 @@ hipError_t launch_forcing(const Phys& ph, int nf, const ForcedMode* modes,
   (void)ph;
   if (nf <= 0) return hipSuccess;
   forcing_k<<<(nf + 255) / 256, 256, 0, s>>>(nf, modes, phases, amp, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_ens_c_pass(int ty, int sched, int mode, const PassArgs* descs,
+                             const uint32_t* items, int nitems, int grid,
+                             hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (ty == 256) return sched ? ens_c_launch<256, 1>(mode, descs, items, nitems, grid, s)
+                              : ens_c_launch<256, 0>(mode, descs, items, nitems, grid, s);
+  if (ty == 512) return sched ? ens_c_launch<512, 1>(mode, descs, items, nitems, grid, s)
+                              : ens_c_launch<512, 0>(mode, descs, items, nitems, grid, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_ens_r_pass(int tx, int ty, int sched, const PassArgs* descs,
+                             const uint32_t* items, int nitems, int grid,
+                             hipStream_t s) {
+  if (nitems <= 0) return hipSuccess;
+  if (tx == 256) return sched ? ens_r_launch<256, 1>(ty, descs, items, nitems, grid, s)
+                              : ens_r_launch<256, 0>(ty, descs, items, nitems, grid, s);
+  if (tx == 512) return sched ? ens_r_launch<512, 1>(ty, descs, items, nitems, grid, s)
+                              : ens_r_launch<512, 0>(ty, descs, items, nitems, grid, s);
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_ka_table(const Phys& ph, const Rect& rc, int kyp, double* ka,
+                           hipStream_t s) {
+  const size_t tot = (size_t)rc.kc() * rc.ky();
+  ka_table_k<<<grid_for(tot, 256), 256, 0, s>>>(ph, rc, kyp, ka);
+  return hipGetLastError();
+}
+
+hipError_t launch_ens_forcing(const EnsForce* f, int nmem, int total,
+                              const int* step, hipStream_t s) {
+  if (total <= 0) return hipSuccess;
+  ens_forcing_k<<<(total + 255) / 256, 256, 0, s>>>(f, nmem, total, step);
+  return hipGetLastError();
+}
+
+hipError_t launch_ens_tick(int* step, hipStream_t s) {
+  ens_tick_k<<<1, 1, 0, s>>>(step);
   return hipGetLastError();
 }
 

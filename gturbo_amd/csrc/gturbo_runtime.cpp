@@ -10,18 +10,12 @@
 #include <unordered_map>
 #include <vector>
 
-#include "gt_internal.hpp"
-#include "gturbo.h"
+#include "gt_runtime.hpp"
 
-using gt::ForcedMode;
-using gt::Phys;
-using gt::Rect;
-
-namespace {
+namespace gtr {
 
 thread_local std::string g_err;
 
-int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -32,92 +26,9 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(call)                                                         \
-  do {                                                                       \
-    hipError_t e_ = (call);                                                  \
-    if (e_ != hipSuccess)                                                    \
-      return fail(GT_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                  __FILE__, __LINE__);                                       \
-  } while (0)
+}  // namespace gtr
 
-#define CHK(call)            \
-  do {                       \
-    int r_ = (call);         \
-    if (r_ != GT_OK) return r_; \
-  } while (0)
-
-constexpr int kNFORC = 2048;  // src/modules.f90:41
-constexpr int kRing = 64;     // forcing-phase slots in flight
-
-struct Mirror {
-  void* host = nullptr;
-  size_t bytes = 0;
-  double* dref = nullptr;   // reference layout
-  bool owned = true;
-  double2* dcmp = nullptr;  // compact layout
-  size_t cmp_cap = 0;       // double2 capacity of dcmp
-  Rect rect{0, 0, 0};
-  int kyp = 0;
-  bool ref_valid = true;
-  bool cmp_valid = false;
-};
-
-struct RectTables {
-  Rect rc{-1, -1, -1};
-  int kyp = 0;
-  double* emk = nullptr;
-  ForcedMode* modes = nullptr;
-};
-
-struct TimedLaunch {
-  int cls;
-  hipEvent_t a, b;
-};
-
-}  // namespace
-
-struct gt_ctx {
-  int n = 0, logn = 0, nh = 0;
-  // commphy / commsim / commforc / commk (src/modules.f90)
-  double xlx, xly, dkx, dky, nu, mu, alpnu, alpmu, alpv;
-  double dt, dt2, dt3, dt6, sdt, inpz = 0, inpe = 0;
-  double famp, k1f, k2f, rtrunc, fkxsmax, fkysmax;
-  int npas, iout, imix, split, rko, truncate;
-  int nf = 0;
-  std::vector<int32_t> ikxf, ikyf, jc;
-  std::vector<double> ff, fkx, fky, fkxs, fkys, emk, emkx, emky;
-
-  Phys ph{};
-  Rect rtr{0, 0, 0}, rfull{0, 0, 0};
-
-  hipStream_t stream = nullptr;
-  double2* tw = nullptr;
-  RectTables tab[2];  // [0] truncated rect, [1] full rect
-  double2* y1 = nullptr;
-  double2* y2 = nullptr;
-  size_t ycap = 0;  // double2 per field
-  int* flag = nullptr;
-
-  // forcing phases
-  double2* ph_host = nullptr;  // pinned, kRing * nf
-  double2* ph_dev = nullptr;
-  hipEvent_t ph_ev[kRing];
-  bool ph_ev_used[kRing] = {};
-  unsigned ph_next = 0;
-
-  // diagnostics
-  double* dg = nullptr;
-  size_t dg_cap = 0;
-
-  // timing
-  bool timing = false;
-  std::vector<TimedLaunch> tl;
-  std::vector<hipEvent_t> ev_pool;
-
-  std::unordered_map<const void*, Mirror> present;
-};
-
-namespace {
+namespace gtr {
 
 int ilog2(int n) {
   int l = 0;
@@ -154,26 +65,22 @@ hipEvent_t get_event(gt_ctx* c) {
   return e;
 }
 
-struct Timer {
-  gt_ctx* c;
-  int cls;
-  hipEvent_t a = nullptr;
-  Timer(gt_ctx* c_, int cls_) : c(c_), cls(cls_) {
-    if (c->timing) {
-      a = get_event(c);
-      if (a) hipEventRecord(a, c->stream);
+Timer::Timer(gt_ctx* c_, int cls_) : c(c_), cls(cls_) {
+  if (c->timing) {
+    a = get_event(c);
+    if (a) hipEventRecord(a, c->stream);
+  }
+}
+
+Timer::~Timer() {
+  if (c->timing && a) {
+    hipEvent_t b = get_event(c);
+    if (b) {
+      hipEventRecord(b, c->stream);
+      c->tl.push_back({cls, a, b});
     }
   }
-  ~Timer() {
-    if (c->timing && a) {
-      hipEvent_t b = get_event(c);
-      if (b) {
-        hipEventRecord(b, c->stream);
-        c->tl.push_back({cls, a, b});
-      }
-    }
-  }
-};
+}
 
 // ---- host tables ----------------------------------------------------------
 
@@ -484,11 +391,13 @@ int same_rect(gt_ctx* c, Mirror& a, Mirror& b) {
   return GT_OK;
 }
 
-}  // namespace
+}  // namespace gtr
 
 // ===========================================================================
 // C ABI
 // ===========================================================================
+
+using namespace gtr;
 
 extern "C" {
 
@@ -545,7 +454,7 @@ int gt_create(int32_t n, const double* p, gt_ctx** out) {
   ph.dkx = c->dkx;
   ph.dky = c->dky;
   ph.ahalf = c->alpv / 2.0;
-  ph.ahalf_is_one = (ph.ahalf == 1.0);
+  ph.akind = ph.ahalf == 1.0 ? 1 : ph.ahalf == 0.5 ? 2 : ph.ahalf == 0.25 ? 3 : 0;
   ph.dt = c->dt;
   ph.dt2 = c->dt2;
   ph.dt3 = c->dt3;

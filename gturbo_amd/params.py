@@ -169,7 +169,11 @@ def workloads() -> dict:
 def ensemble_members() -> list:
     """64 members: 32 at 256^2, 24 at 512^2, 8 at 1024^2.  Member m has
     seed 100+m, alpha cycling through {0.5, 1, 2}, ring (N/16-1, N/16+1],
-    nu = (N/256)**-8 * 1e-16 (k^8), mu = 0.1, dt = 0.25/N, 1000 steps."""
+    nu = (N/256)**-8 * 1e-16 (k^8), mu = 0.1, dt = 0.25/N, 1000 steps.
+    The forcing amplitude is not part of the workload's definition: famp =
+    0.1 keeps every member stable over the 1000 steps (at famp = 1 the
+    alpha = 1/2 members blow up near step 500 at this dt, in the oracle as
+    on the GPU)."""
     out = []
     sizes = [256] * 32 + [512] * 24 + [1024] * 8
     for m, n in enumerate(sizes):
@@ -177,7 +181,7 @@ def ensemble_members() -> list:
         alpv = (0.5, 1.0, 2.0)[m % 3]
         p = Params(n=n, nu=(n / 256.0) ** -8 * 1e-16, alpnu=4.0, mu=0.1, alpmu=0.0,
                    alpv=alpv, dt=0.25 / n, rko=4, npas=1000, iout=10**9,
-                   imix=10**9, famp=1.0, k1f=k1, k2f=k2)
+                   imix=10**9, famp=0.1, k1f=k1, k2f=k2)
         ic = InitialSpectrum(seed=100 + m, p=4.0 if alpv == 2.0 else 2.0,
                              k0=n / 32.0, energy=0.5)
         out.append(Workload(f"member{m}: N={n} alpha={alpv}", p, ic, 481929 + m))

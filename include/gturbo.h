@@ -146,6 +146,56 @@ int gt_timing_enable(gt_ctx *ctx, int32_t on);
  * and launch counts; resets the sums. */
 int gt_timing_read(gt_ctx *ctx, double *ms, int32_t *count);
 
+/* --- batched ensembles ---------------------------------------------------
+ * The reference runs an ensemble as copies of one run directory, each
+ * submitted as its own job (src/reset.sh:12-77, src/jobscript:14), each job
+ * the main loop of src/main.f90:97-137.  Here the members -- each with its
+ * own params.dat values, N (128 to 2048), field and forcing seed -- advance
+ * together: every pass of the RK step is one launch for all members, and the
+ * whole step (passes, then the forcing of every member, src/main.f90:99-136
+ * without the diagnostics) is captured in a HIP graph and replayed.  All
+ * members use the same RK order and the 2/3 rule (truncate = 1). */
+typedef struct gt_ens gt_ens;
+
+/* Field spaces of gt_ens_set_field / gt_ens_get_field. */
+#define GT_SPACE_PHYSICAL 0 /* z(N+2,N) in physical space (src/read.f90) */
+#define GT_SPACE_SPECTRAL 1 /* complex z(N/2+1,N), reference layout */
+
+/* Schedule of the batched passes (gt_ens_schedule, gt_ens_tune):
+ * [0] threads per y-pass workgroup (256 or 512; a workgroup holds that many
+ *     times 8/N kx-columns), [1] threads per x-pass workgroup (256 or 512),
+ * [2] waves per SIMD of the persistent grid (1, 2 or 4; 0 = one workgroup
+ *     per work item), [3] radix order of the transforms (0 = radix 8 first,
+ *     1 = the radix-2/4 pass first), [4] bit L set: members with N = 2^L
+ *     use the row-pair x pass, [5] RK steps per captured graph (1-256). */
+#define GT_ENS_NCFG 6
+
+/* nmem members; n[m] = N of member m; params holds nmem rows of the 17
+ * params.dat values of gt_create.  Members start from a zero field and the
+ * forcing seed 481929 (src/seed.0). */
+int gt_ens_create(int32_t nmem, const int32_t *n, const double *params, gt_ens **ens);
+int gt_ens_destroy(gt_ens *ens);
+/* Member m's field from a host array of (N+2)*N doubles.  GT_SPACE_PHYSICAL
+ * applies fft_dir and trunc as src/main.f90:89-92 does to the input frame;
+ * GT_SPACE_SPECTRAL takes the spectrum as is (it must vanish outside the
+ * 2/3 rectangle). */
+int gt_ens_set_field(gt_ens *ens, int32_t m, const double *z, int32_t space);
+/* Member m's field into a host array of (N+2)*N doubles (GT_SPACE_PHYSICAL:
+ * after fft_inv, as src/main.f90:143 before writef). */
+int gt_ens_get_field(gt_ens *ens, int32_t m, double *z, int32_t space);
+/* The forcing LCG seed of member m (src/forcing.f90:40-51). */
+int gt_ens_set_seed(gt_ens *ens, int32_t m, int32_t seed);
+int gt_ens_get_seed(gt_ens *ens, int32_t m, int32_t *seed);
+/* set = 0: read the schedule into cfg; set = 1: use cfg. */
+int gt_ens_schedule(gt_ens *ens, int32_t *cfg, int32_t set);
+/* Autotuner: times candidate schedules over `steps` RK steps of this
+ * ensemble (fields and seeds are left as they were), keeps the fastest and
+ * returns it in cfg with its time per RK step (ms, forcing excluded). */
+int gt_ens_tune(gt_ens *ens, int32_t steps, int32_t *cfg, double *ms_per_step);
+/* nsteps x (RK step of every member, then its forcing).  Asynchronous. */
+int gt_ens_run(gt_ens *ens, int32_t nsteps);
+int gt_ens_synchronize(gt_ens *ens);
+
 #ifdef __cplusplus
 }
 #endif
