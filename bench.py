@@ -12,6 +12,13 @@ kernel class with the largest total time, priced at its algorithmic bytes.
 The CPU baseline is the oracle (plain C, OpenMP) timed on this host on a
 bounded sample of the same workload.
 
+Also measured, in the same line: the RK2 variant of the headline workload
+("rk2"); the batched ensemble ("ensemble": 32 x 256^2, 24 x 512^2,
+8 x 1024^2, alpha cycling through 1/2, 1, 2) against the plain per-run loop
+of the same members, both over the same number of steps; and the cost of
+one adjoint gradient against the plain forward run ("adjoint": N=512,
+alpha=1, 200 RK4 steps, checkpoints every 20).
+
 Prints ONE JSON line.
 """
 from __future__ import annotations
@@ -93,6 +100,140 @@ def cpu_baseline(p, w0, seed, budget_s=10.0, max_steps=40):
                       f"({el:.1f} s wall, {threads} OpenMP threads)"}
 
 
+def rk_rate(rko, n, steps, warmup):
+    """grid-pt-steps/s of step(+forcing) for the headline workload at RK order
+    rko (the same loop as the headline measurement)."""
+    from gturbo_amd import ic, io, workloads
+    from gturbo_amd.solver import Solver
+    w = workloads()[f"euler4096_rk{rko}"]
+    p = w.params.replace(n=n)
+    with Solver(p) as s:
+        z = io.field_to_ref(ic.physical_field(p, w.ic))
+        arrs = [z] + [s.field() for _ in range(4)]
+        s.enter_data(z)
+        for a in arrs[1:]:
+            s.enter_data(a, copyin=False)
+        s.fft_dir(z)
+        s.trunc(z)
+        seed = w.forcing_seed
+        for it in range(warmup + steps):
+            if it == warmup:
+                s.synchronize()
+                t0 = time.perf_counter()
+            if rko == 4:
+                s.step4(*arrs)
+            else:
+                s.step(arrs[0], arrs[1], arrs[3], arrs[4])
+            seed = s.forcing(z, seed)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        s.update_host(z)
+        ok = bool(np.all(np.isfinite(z)))
+    return {"workload": w.name, "value": n * n * steps / el, "unit": "grid-pt-steps/s",
+            "ms_per_step": el / steps * 1e3, "steps": steps, "finite": ok}
+
+
+def ensemble_bench(steps):
+    """Batched ensemble (tuned schedule, graph-captured steps) against the
+    plain per-run loop (each member alone: step4 + forcing per step through
+    the single-run entry points), same members, same number of steps."""
+    from gturbo_amd import ensemble as E, ic
+    from gturbo_amd.params import ensemble_members
+    from gturbo_amd.solver import Solver
+    wl = ensemble_members()
+    fields = [ic.physical_field(w.params, w.ic) for w in wl]
+    ens = E.Ensemble([w.params for w in wl], seeds=[w.forcing_seed for w in wl])
+    for m, f in enumerate(fields):
+        ens.set_field(m, f)
+    cfg, _ = ens.tune(3)
+    ens.run(10)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.run(steps)
+    ens.synchronize()
+    tb = time.perf_counter() - t0
+    finite = all(bool(np.all(np.isfinite(ens.get_field(m, E.SPECTRAL)))) for m in range(len(wl)))
+    ens.close()
+    tp = 0.0
+    for w, f in zip(wl, fields):
+        p = w.params
+        with Solver(p) as s:
+            z = np.zeros((p.n, p.n + 2))
+            z[:, :p.n] = f
+            arrs = [z] + [np.zeros_like(z) for _ in range(4)]
+            for a in arrs:
+                s.enter_data(a)
+            s.fft_dir(z)
+            s.trunc(z)
+            seed = w.forcing_seed
+            for it in range(3 + steps):
+                if it == 3:
+                    s.synchronize()
+                    t0 = time.perf_counter()
+                s.step4(*arrs)
+                seed = s.forcing(z, seed)
+            s.synchronize()
+            tp += time.perf_counter() - t0
+            for a in arrs:
+                s.exit_data(a, copyout=False)
+    gp = sum(w.params.n ** 2 for w in wl)
+    sizes = {}
+    for w in wl:
+        sizes[w.params.n] = sizes.get(w.params.n, 0) + 1
+    return {"workload": "+".join(f"{c}x{n}^2" for n, c in sorted(sizes.items())) +
+                        " RK4, alpha 1/2,1,2, forced, dt=0.25/N",
+            "members": len(wl), "steps": steps, "schedule": cfg,
+            "batched_ms_per_step": tb / steps * 1e3,
+            "plain_loop_ms_per_step": tp / steps * 1e3,
+            "speedup": tp / tb,
+            "batched_grid_pt_steps_per_s": gp * steps / tb,
+            "finite": finite}
+
+
+def adjoint_bench():
+    """The adjoint workload (N=512, alpha=1, RK4, dt=5e-4, 200 steps, forcing
+    off, checkpoints every 20 steps): time of gt_adj_forward + gt_adj_backward
+    (one gradient) against the plain forward loop of the same steps."""
+    from gturbo_amd import ic, workloads
+    from gturbo_amd.adjoint import Adjoint
+    from gturbo_amd.solver import Solver
+    w = workloads()["adjoint512"]
+    p = w.params
+    n, steps = p.n, p.npas
+    w0 = ic.physical_field(p, w.ic).astype(np.float64)
+    with Solver(p) as s:
+        z = np.zeros((n, n + 2))
+        z[:, :n] = w0
+        arrs = [z] + [s.field() for _ in range(4)]
+        s.enter_data(z)
+        for a in arrs[1:]:
+            s.enter_data(a, copyin=False)
+        s.fft_dir(z)
+        s.trunc(z)
+        seed = w.forcing_seed
+        for it in range(5 + steps):
+            if it == 5:
+                s.synchronize()
+                t0 = time.perf_counter()
+            s.step4(*arrs)
+            seed = s.forcing(z, seed)
+        s.synchronize()
+        t_plain = time.perf_counter() - t0
+    with Adjoint(p, steps, kchk=20) as adj:
+        adj.forward(w0, w.forcing_seed)  # warm-up (allocations)
+        adj.backward(np.zeros((n, n)))
+        t0 = time.perf_counter()
+        wT, _ = adj.forward(w0, w.forcing_seed)
+        t1 = time.perf_counter()
+        g = adj.backward(wT)
+        t2 = time.perf_counter()
+    return {"workload": w.name + ", checkpoints every 20 steps",
+            "plain_forward_ms": t_plain * 1e3, "adjoint_forward_ms": (t1 - t0) * 1e3,
+            "adjoint_backward_ms": (t2 - t1) * 1e3,
+            "gradient_cost_vs_forward": (t2 - t0) / t_plain,
+            "finite": bool(np.all(np.isfinite(g)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +242,12 @@ def main():
     ap.add_argument("--rko", type=int, default=4, choices=(2, 4))
     ap.add_argument("--n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rk2-steps", type=int, default=50,
+                    help="steps of the RK2 variant of the headline workload (0: skip)")
+    ap.add_argument("--ens-steps", type=int, default=100,
+                    help="steps of the batched-ensemble comparison (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint cost measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device-to-device copy of a reference-layout "
                          "field (a plain copy of known size for HBM counter scaling)")
@@ -204,6 +351,13 @@ def main():
                        "hbm_GBs_at_measured_rate": step_bytes * args.steps / el / 1e9},
         "kernels": kernels,
     }
+    s.close()
+    if args.rk2_steps > 0 and args.rko == 4:
+        out["rk2"] = rk_rate(2, args.n, args.rk2_steps, 5)
+    if args.ens_steps > 0:
+        out["ensemble"] = ensemble_bench(args.ens_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, w0, w.forcing_seed)
     print(json.dumps(out))

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgturbo_hip.so")
@@ -47,6 +48,8 @@ SIGNATURES = {
     "gt_shell_index": (C.c_int, [_P, _P]),
     "gt_timing_enable": (C.c_int, [_P, _I32]),
     "gt_timing_read": (C.c_int, [_P, _P, _P]),
+    "gt_adj_forward": (C.c_int, [_P, _P, _P, _I32, _I32, C.POINTER(_I32)]),
+    "gt_adj_backward": (C.c_int, [_P, _P, _P]),
     "gt_ens_create": (C.c_int, [_I32, _P, _P, C.POINTER(_P)]),
     "gt_ens_destroy": (C.c_int, [_P]),
     "gt_ens_set_field": (C.c_int, [_P, _I32, _P, _I32]),
@@ -60,6 +63,11 @@ SIGNATURES = {
 }
 
 _lib = None
+# Whether torch was imported before the library was loaded.  torch ships its
+# own HIP runtime; the library binds to it only if it is already loaded, so
+# device tensors can be shared with torch only in that order (otherwise the
+# process holds two runtimes and torch finds no GPU).
+TORCH_FIRST = False
 
 
 class GturboError(RuntimeError):
@@ -68,8 +76,9 @@ class GturboError(RuntimeError):
 
 def load():
     """Load the library (once).  Raises if it has not been built."""
-    global _lib
+    global _lib, TORCH_FIRST
     if _lib is None:
+        TORCH_FIRST = "torch" in sys.modules
         if not os.path.exists(LIB_PATH):
             raise GturboError(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() or "

@@ -75,6 +75,7 @@ def build_library(verbose=False, jobs=None):
     units += [(os.path.join(OBJ, f"k{l}.o"), [src_k, f"-DGT_INST_LOGN={l}"]) for l in LOGNS]
     units.append((os.path.join(OBJ, "runtime.o"), [os.path.join(CSRC, "gturbo_runtime.cpp")]))
     units.append((os.path.join(OBJ, "ensemble.o"), [os.path.join(CSRC, "gturbo_ensemble.cpp")]))
+    units.append((os.path.join(OBJ, "adjoint.o"), [os.path.join(CSRC, "gturbo_adjoint.cpp")]))
     jobs = jobs or min(16, os.cpu_count() or 4)
 
     def one(u):
@@ -91,25 +92,30 @@ def build_library(verbose=False, jobs=None):
 
 
 def build_fortran(verbose=False):
+    """gturbo_run (the driver of src/main.f90) and gturbo_invert (initial-
+    condition inversion through the adjoint), both on the library."""
     fc = flang()
     if fc is None:
         if verbose:
-            print("no Fortran compiler: skipping the Fortran driver")
+            print("no Fortran compiler: skipping the Fortran programs")
         return None
-    srcs = ["gturbo_hip.f90", "gturbo_state.f90", "gturbo_sub.f90", "gturbo_io.f90",
-            "main.f90"]
-    paths = [os.path.join(FDIR, s) for s in srcs if os.path.exists(os.path.join(FDIR, s))]
-    if len(paths) != len(srcs):
+    common = ["gturbo_hip.f90", "gturbo_state.f90", "gturbo_sub.f90", "gturbo_io.f90"]
+    programs = {"gturbo_run": "main.f90", "gturbo_invert": "invert.f90"}
+    srcs = common + list(programs.values())
+    if not all(os.path.exists(os.path.join(FDIR, s)) for s in srcs):
         return None
     mod = os.path.join(OBJ, "fmod")
     os.makedirs(mod, exist_ok=True)
-    objs = []
-    for src in paths:  # in order: modules before their users
-        obj = os.path.join(mod, os.path.basename(src) + ".o")
-        _run([fc, "-O2", "-c", "-module-dir", mod, f"-I{mod}", src, "-o", obj], verbose)
-        objs.append(obj)
-    _run([fc, "-O2", *objs, "-o", FEXE, f"-L{HERE}", "-lgturbo_hip",
-          "-Wl,-rpath,$ORIGIN/.."], verbose)
+    objs = {}
+    for src in srcs:  # in order: modules before their users
+        obj = os.path.join(mod, src + ".o")
+        _run([fc, "-O2", "-c", "-module-dir", mod, f"-I{mod}", os.path.join(FDIR, src), "-o",
+              obj], verbose)
+        objs[src] = obj
+    for exe, main in programs.items():
+        _run([fc, "-O2", *[objs[s] for s in common], objs[main], "-o",
+              os.path.join(FDIR, exe), f"-L{HERE}", "-lgturbo_hip",
+              "-Wl,-rpath,$ORIGIN/.."], verbose)
     return FEXE
 
 

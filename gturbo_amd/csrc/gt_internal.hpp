@@ -46,9 +46,39 @@ struct PassArgs {
   const double* emk;  // integrating factor (compact, real)
   double2* out;       // nonlinear term (compact), NLT mode
   const double* ka;   // k2**(alpv/2) (compact), batched passes with akind 0
+  double2* sst;       // if set: the stage input formed by this y pass is
+                      // also stored here (compact), for the adjoint's recompute
   // batched ensemble passes only: x-pass packing (0 = one row per line,
   // 1 = row pairs)
   int xvar;
+};
+
+// Arguments of the adjoint passes (transposed nonlinear term inside the
+// reversed RK stages).
+struct AdjArgs {
+  Rect rc;
+  int kyp;
+  int ntiles;
+  Phys ph;
+  const double2* tw;
+  double2* y1;           // 5 fields after the inverse y transforms:
+                         // lambda_x, lambda_y, q, u, v
+  double2* y2;           // 3 fields after the forward x transforms:
+                         // u lambda_x + v lambda_y, q lambda_x, q lambda_y
+  double2* lam;          // adjoint state: lambda' in, lambda out (J_END)
+  double2* lacc;         // the step's adjoint accumulator
+  const double* emk;
+  const double2* snext;  // stage state of the next transposed term
+};
+
+enum JMode : int {
+  J_START4 = 0,  // RK4: lacc = E^2 mu, a = dt/6 mu; inverse part for s4
+  J_RK4_3 = 1,   // a_s4 in: lacc += E^2 a_s4, a = dt/3 E mu + dt E a_s4 (s3)
+  J_RK4_2 = 2,   // a_s3 in: lacc += E a_s3, a = dt/3 E mu + dt/2 a_s3 (s2)
+  J_RK4_1 = 3,   // a_s2 in: lacc += E a_s2, a = dt/6 E^2 mu + dt/2 E a_s2 (s1)
+  J_END = 4,     // a_s1 in: lambda = lacc + a_s1
+  J_START2 = 5,  // RK2: lacc = E^2 mu, a = dt E mu (s2)
+  J_RK2_1 = 6,   // a_s2 in: lacc += E a_s2, a = dt/2 E a_s2 (s1)
 };
 
 enum CMode : int {
@@ -134,6 +164,8 @@ hipError_t launch_c_pass(int logn, int mode, const PassArgs& a, hipStream_t s);
 hipError_t launch_r_pass(int logn, const PassArgs& a, hipStream_t s);
 hipError_t launch_ref_fft(int logn, int dir, double* z, const double2* tw,
                           hipStream_t s);
+hipError_t launch_j_pass(int logn, int mode, const AdjArgs& a, hipStream_t s);
+hipError_t launch_jr_pass(int logn, const AdjArgs& a, hipStream_t s);
 hipError_t launch_produ(int n, const double* z, double* u, double* v,
                         hipStream_t s);
 hipError_t launch_trunc_ref(const Phys& ph, double* z, hipStream_t s);

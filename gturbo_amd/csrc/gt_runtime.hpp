@@ -65,6 +65,8 @@ struct TimedLaunch {
 
 }  // namespace gtr
 
+struct AdjState;  // gturbo_adjoint.cpp
+
 struct gt_ctx {
   int n = 0, logn = 0, nh = 0;
   // commphy / commsim / commforc / commk (src/modules.f90)
@@ -104,6 +106,8 @@ struct gt_ctx {
   std::vector<hipEvent_t> ev_pool;
 
   std::unordered_map<const void*, gtr::Mirror> present;
+
+  AdjState* adj = nullptr;  // checkpoints and buffers of gt_adj_*
 };
 
 namespace gtr {
@@ -123,6 +127,8 @@ gt::PassArgs pass_args(gt_ctx* c, Mirror& z, RectTables* t);
 int c_pass(gt_ctx* c, int mode, const gt::PassArgs& a);
 int r_pass(gt_ctx* c, const gt::PassArgs& a);
 void forcing_phases(int nf, int32_t* seed, double2* out);
+int forcing_on(gt_ctx* c, RectTables* t, double2* z, int32_t* seed);
+void adj_free(gt_ctx* c);  // gturbo_adjoint.cpp
 
 // Timer of one launch class on the context's stream (GT_K_* of gturbo.h).
 struct Timer {

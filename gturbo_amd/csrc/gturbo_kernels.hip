@@ -228,6 +228,13 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
   }
 
   if constexpr (INV) {
+    if (a.sst != nullptr) {  // stage input kept for the adjoint's recompute
+#pragma unroll
+      for (int r = 0; r < kE; ++r) {
+        const int row = row_of(j + TPL * r, rc, N);
+        if (live_r<TR>(r) && colok && row >= 0) a.sst[cbase + (unsigned)row] = s[r];
+      }
+    }
     // ka is tabled only for AK = 0; for AK = 1 it is k2 and for AK = 2 it is
     // evaluated where needed.
     double ka[AK == 0 ? kE : 1];
@@ -548,6 +555,241 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE)
 }
 
 // ---------------------------------------------------------------------------
+// Adjoint passes.  With q, u, v the physical fields of a stage state s and
+// lambda_x,y = G(i k_x,y a) those of the adjoint input a (G the unscaled
+// inverse transform, F = G^-1), the transpose of the linearised nonlinear
+// term in the Parseval inner product is
+//   J(s)^T a = P [ F(u lambda_x + v lambda_y)
+//                  + (-i k_y F(q lambda_x) + i k_x F(q lambda_y)) / |k|^alpha ]
+// (P the 2/3 mask, the mean mode zero).  The adjoint y pass holds the
+// forward y transforms of the three products and their combination, the
+// reversed RK stage (JMode) and the inverse y transforms of lambda_x,
+// lambda_y, q, u, v for the next transposed term; the adjoint x pass forms
+// the products.
+// ---------------------------------------------------------------------------
+
+template <int LOGN, int MODE, int AK, int TR>
+__global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 1)
+    j_pass(AdjArgs a) {
+  constexpr int N = 1 << LOGN, TPL = N / kE, BT = bt_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED;
+  constexpr bool FWD = MODE != J_START4 && MODE != J_START2;
+  constexpr bool INV = MODE != J_END;
+  __shared__ double2 lds[BT * LINE];
+  int b, j;
+  line_map<TPL, BT>(threadIdx.x, b, j);
+  const int tile = blockIdx.x;
+  const int c = tile * BT + b;
+  const Rect rc = a.rc;
+  const bool colok = c <= rc.kx;
+  double2* buf = lds + b * LINE;
+  const Phys& ph = a.ph;
+  const double fkx = (double)c * ph.dkx;
+  const double fkxs = fkx * fkx;
+  const unsigned cbase = (unsigned)c * (unsigned)a.kyp;
+  double2 z[kE];
+  double2 av[kE];  // adjoint input of the next transposed term
+
+  if constexpr (FWD) {
+    double2 C[kE];
+    const double2* y2 = a.y2 + (size_t)tile * N * BT;
+    const size_t fs = (size_t)a.ntiles * N * BT;
+#pragma unroll
+    for (int r = 0; r < kE; ++r) z[r] = y2[fs + (unsigned)((j + TPL * r) * BT + b)];
+    fft_reg<LOGN, -1>(z, buf, j, a.tw);  // F(q lambda_x)
+#pragma unroll
+    for (int r = 0; r < kE; ++r) C[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int r = 0; r < kE; ++r) z[r] = y2[2 * fs + (unsigned)((j + TPL * r) * BT + b)];
+    fft_reg<LOGN, -1>(z, buf, j, a.tw);  // F(q lambda_y)
+    int jc = j;
+    asm volatile("" : "+v"(jc) : "v"(z[0].x), "v"(z[kE - 1].y));
+#pragma unroll
+    for (int r = 0; r < kE; ++r) {
+      const int p = jc + TPL * r;
+      double2 v = make_double2(0.0, 0.0);
+      if (live_r<TR>(r) && colok && row_of(p, rc, N) >= 0 && !(c == 0 && p == 0)) {
+        const double fky = (double)kyi_of(p, N) * ph.dky;
+        const double k2 = fkxs + fky * fky;
+        const double kar = AK == 1 ? k2 : kalpha(ph, k2);
+        v = make_double2((fky * C[r].y - fkx * z[r].y) / kar,
+                         (fkx * z[r].x - fky * C[r].x) / kar);
+      }
+      C[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < kE; ++r) z[r] = y2[(unsigned)((j + TPL * r) * BT + b)];
+    fft_reg<LOGN, -1>(z, buf, j, a.tw);  // F(u lambda_x + v lambda_y)
+    int je = j;
+    asm volatile("" : "+v"(je) : "v"(z[0].x), "v"(z[kE - 1].y));
+#pragma unroll
+    for (int r = 0; r < kE; ++r) {
+      const int p = je + TPL * r;
+      const int row = row_of(p, rc, N);
+      av[r] = make_double2(0.0, 0.0);
+      if (live_r<TR>(r) && colok && row >= 0) {
+        const double fky = (double)kyi_of(p, N) * ph.dky;
+        const bool mean = (c == 0 && p == 0);
+        // the transposed term (fft_dir's 1/N^2 on the forward transforms)
+        double2 t = make_double2((z[r].x + C[r].x) * ph.inv_n2, (z[r].y + C[r].y) * ph.inv_n2);
+        if ((ph.truncate == 1 && cut_mode(ph, fkxs, fky * fky)) || mean)
+          t = make_double2(0.0, 0.0);
+        const unsigned q = cbase + (unsigned)row;
+        const double E = a.emk[q];
+        if constexpr (MODE == J_END) {
+          const double2 l = a.lacc[q];
+          a.lam[q] = make_double2(l.x + t.x, l.y + t.y);
+        } else {
+          const double2 mu = mean ? make_double2(0.0, 0.0) : a.lam[q];
+          double2 l = a.lacc[q];
+          if constexpr (MODE == J_RK4_3) {
+            const double EE = E * E;
+            l = make_double2(l.x + EE * t.x, l.y + EE * t.y);
+            av[r] = make_double2(ph.dt3 * E * mu.x + ph.dt * E * t.x,
+                                 ph.dt3 * E * mu.y + ph.dt * E * t.y);
+          } else if constexpr (MODE == J_RK4_2) {
+            l = make_double2(l.x + E * t.x, l.y + E * t.y);
+            av[r] = make_double2(ph.dt3 * E * mu.x + ph.dt2 * t.x,
+                                 ph.dt3 * E * mu.y + ph.dt2 * t.y);
+          } else if constexpr (MODE == J_RK4_1) {
+            const double EE = E * E;
+            l = make_double2(l.x + E * t.x, l.y + E * t.y);
+            av[r] = make_double2(ph.dt6 * EE * mu.x + ph.dt2 * E * t.x,
+                                 ph.dt6 * EE * mu.y + ph.dt2 * E * t.y);
+          } else if constexpr (MODE == J_RK2_1) {
+            l = make_double2(l.x + E * t.x, l.y + E * t.y);
+            av[r] = make_double2(ph.dt2 * E * t.x, ph.dt2 * E * t.y);
+          }
+          a.lacc[q] = l;
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kE; ++r) {
+      const int p = j + TPL * r;
+      const int row = row_of(p, rc, N);
+      av[r] = make_double2(0.0, 0.0);
+      if (live_r<TR>(r) && colok && row >= 0) {
+        const unsigned q = cbase + (unsigned)row;
+        const double2 mu = (c == 0 && p == 0) ? make_double2(0.0, 0.0) : a.lam[q];
+        const double E = a.emk[q];
+        const double EE = E * E;
+        a.lacc[q] = make_double2(EE * mu.x, EE * mu.y);
+        if constexpr (MODE == J_START4) av[r] = make_double2(ph.dt6 * mu.x, ph.dt6 * mu.y);
+        else av[r] = make_double2(ph.dt * E * mu.x, ph.dt * E * mu.y);
+      }
+    }
+  }
+
+  if constexpr (INV) {
+#pragma unroll 1
+    for (int f = 0; f < 5; ++f) {
+      int jo = j;
+      asm volatile("" : "+v"(jo));
+#pragma unroll
+      for (int r = 0; r < kE; ++r) {
+        const int p = jo + TPL * r;
+        const int row = row_of(p, rc, N);
+        double2 v = make_double2(0.0, 0.0);
+        if (live_r<TR>(r) && colok && row >= 0) {
+          const double fky = (double)kyi_of(p, N) * ph.dky;
+          if (f == 0) {
+            v = make_double2(-fkx * av[r].y, fkx * av[r].x);  // i kx a
+          } else if (f == 1) {
+            v = make_double2(-fky * av[r].y, fky * av[r].x);  // i ky a
+          } else {
+            const double2 sv = a.snext[cbase + (unsigned)row];
+            if (f == 2) {
+              v = sv;
+            } else if (!(c == 0 && p == 0)) {  // u, v as src/nlt.f90:25-34
+              const double k2 = fkxs + fky * fky;
+              const double kar = AK == 1 ? k2 : kalpha(ph, k2);
+              if (f == 3) {
+                const double kyn = fky / kar;
+                v = make_double2(-kyn * sv.y, kyn * sv.x);
+              } else {
+                const double kxn = fkx / kar;
+                v = make_double2(kxn * sv.y, -kxn * sv.x);
+              }
+            }
+          }
+        }
+        z[r] = v;
+      }
+      fft_reg<LOGN, +1>(z, buf, j, a.tw);
+      double2* dst = a.y1 + (size_t)(f * a.ntiles + tile) * N * BT;
+#pragma unroll
+      for (int r = 0; r < kE; ++r) dst[(unsigned)((j + TPL * r) * BT + b)] = z[r];
+    }
+  }
+}
+
+// Half-spectrum (kx <= kx_max) of one real row -> its full x-spectrum at
+// the thread's positions (as build_two with g = 0).
+template <int LOGN, int TR, int BT>
+__device__ __forceinline__ void build_one(double2* z, int j, const double2* __restrict__ y,
+                                          unsigned off, int kxm) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
+#pragma unroll
+  for (int r = 0; r < kE; ++r) {
+    const int p = j + TPL * r;
+    double2 v = make_double2(0.0, 0.0);
+    const int k = p <= N / 2 ? p : N - p;
+    if (live_r<TR>(r) && k <= kxm) {
+      const double2 x0 = y[off + koff<LOGN, BT>(k)];
+      if (p == 0 || p == N / 2) v = make_double2(x0.x, 0.0);
+      else if (p < N / 2) v = x0;
+      else v = make_double2(x0.x, -x0.y);
+    }
+    z[r] = v;
+  }
+}
+
+// Adjoint x pass, one row per line: C2R of lambda_x + i lambda_y, of q + i u
+// and of v; R2C of q lambda_x + i q lambda_y and of u lambda_x + v lambda_y.
+template <int LOGN, int TR>
+__global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
+    jr_pass(AdjArgs a) {
+  constexpr int N = 1 << LOGN, TPL = N / kE, RP = rows_of(LOGN), BT = bt_of(LOGN);
+  constexpr int LINE = Line<LOGN>::PADDED;
+  __shared__ double2 lds[RP * LINE];
+  int pl, j;
+  line_map<TPL, RP>(threadIdx.x, pl, j);
+  const int yrow = xcd_remap(blockIdx.x, gridDim.x) * RP + pl;
+  double2* buf = lds + pl * LINE;
+  const int kxm = a.rc.kx;
+  const unsigned fs = (unsigned)a.ntiles * N * BT;
+  const unsigned off = (unsigned)yrow * BT;
+
+  double2 z[kE], l[kE];
+  double ap[kE];
+  build_two<LOGN, TR, BT>(z, j, a.y1, off, off + fs, kxm);  // LX + i LY
+  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) l[r] = z[r];                   // lambda_x + i lambda_y
+  build_two<LOGN, TR, BT>(z, j, a.y1, off + 2 * fs, off + 3 * fs, kxm);  // Q + i U
+  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) {
+    ap[r] = z[r].y * l[r].x;                                  // u lambda_x
+    z[r] = make_double2(z[r].x * l[r].x, z[r].x * l[r].y);    // q lambda_x + i q lambda_y
+  }
+  fft_reg<LOGN, -1>(z, buf, j, a.tw);
+  split_store<LOGN, TR, BT>(z, buf, j, a.y2, off + fs, off + 2 * fs, kxm);
+  build_one<LOGN, TR, BT>(z, j, a.y1, off + 4 * fs, kxm);   // V
+  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) z[r] = make_double2(ap[r] + z[r].x * l[r].y, 0.0);
+  fft_reg<LOGN, -1>(z, buf, j, a.tw);                          // F(u lambda_x + v lambda_y)
+#pragma unroll
+  for (int r = 0; r <= kE / 2; ++r) {
+    const int k = j + TPL * r;
+    if (live_r<TR>(r) && k <= kxm && k <= N / 2) a.y2[off + koff<LOGN, BT>(k)] = z[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // launchers.  Each grid size is instantiated in its own translation unit
 // (compiled with -DGT_INST_LOGN=L); the dispatchers and the size-independent
 // kernels are in the one compiled with -DGT_INST_COMMON.
@@ -559,6 +801,10 @@ template <int L>
 hipError_t r_pass_launch(const PassArgs& a, hipStream_t s);
 template <int L>
 hipError_t ref_fft_launch(int dir, double* z, const double2* tw, hipStream_t s);
+template <int L>
+hipError_t j_pass_launch(int mode, const AdjArgs& a, hipStream_t s);
+template <int L>
+hipError_t jr_pass_launch(const AdjArgs& a, hipStream_t s);
 template <int T, int SCHED>
 hipError_t ens_c_launch(int mode, const PassArgs* d, const uint32_t* it, int n, int grid,
                         hipStream_t s);
@@ -633,6 +879,41 @@ hipError_t ref_fft_launch(int dir, double* z, const double2* tw, hipStream_t s) 
   return hipGetLastError();
 }
 
+template <int L, int AK>
+static hipError_t j_pass_launch_ak(int mode, const AdjArgs& a, hipStream_t s) {
+  constexpr int T = bt_of(L) * (1 << L) / kE;
+  const dim3 g(a.ntiles), b(T);
+  switch (mode) {
+    case J_START4: j_pass<L, J_START4, AK, 1><<<g, b, 0, s>>>(a); break;
+    case J_RK4_3: j_pass<L, J_RK4_3, AK, 1><<<g, b, 0, s>>>(a); break;
+    case J_RK4_2: j_pass<L, J_RK4_2, AK, 1><<<g, b, 0, s>>>(a); break;
+    case J_RK4_1: j_pass<L, J_RK4_1, AK, 1><<<g, b, 0, s>>>(a); break;
+    case J_END: j_pass<L, J_END, AK, 1><<<g, b, 0, s>>>(a); break;
+    case J_START2: j_pass<L, J_START2, AK, 1><<<g, b, 0, s>>>(a); break;
+    case J_RK2_1: j_pass<L, J_RK2_1, AK, 1><<<g, b, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// The adjoint needs the 2/3 rectangle (TR = 1).
+template <int L>
+hipError_t j_pass_launch(int mode, const AdjArgs& a, hipStream_t s) {
+  if (!rect_tr(a.rc, 1 << L)) return hipErrorInvalidValue;
+  return a.ph.akind == 1 ? j_pass_launch_ak<L, 1>(mode, a, s)
+                         : j_pass_launch_ak<L, 0>(mode, a, s);
+}
+
+template <int L>
+hipError_t jr_pass_launch(const AdjArgs& a, hipStream_t s) {
+  if (!rect_tr(a.rc, 1 << L)) return hipErrorInvalidValue;
+  constexpr int RP = rows_of(L), T = RP * (1 << L) / kE;
+  jr_pass<L, 1><<<dim3((1 << L) / RP), dim3(T), 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template hipError_t j_pass_launch<GT_INST_LOGN>(int, const AdjArgs&, hipStream_t);
+template hipError_t jr_pass_launch<GT_INST_LOGN>(const AdjArgs&, hipStream_t);
 template hipError_t c_pass_launch<GT_INST_LOGN>(int, const PassArgs&, hipStream_t);
 template hipError_t r_pass_launch<GT_INST_LOGN>(const PassArgs&, hipStream_t);
 template hipError_t ref_fft_launch<GT_INST_LOGN>(int, double*, const double2*, hipStream_t);
@@ -1109,6 +1390,16 @@ hipError_t launch_c_pass(int logn, int mode, const PassArgs& a, hipStream_t s) {
 
 hipError_t launch_r_pass(int logn, const PassArgs& a, hipStream_t s) {
   GT_LOGN_SWITCH(logn, return r_pass_launch<L>(a, s));
+  return hipSuccess;
+}
+
+hipError_t launch_j_pass(int logn, int mode, const AdjArgs& a, hipStream_t s) {
+  GT_LOGN_SWITCH(logn, return j_pass_launch<L>(mode, a, s));
+  return hipSuccess;
+}
+
+hipError_t launch_jr_pass(int logn, const AdjArgs& a, hipStream_t s) {
+  GT_LOGN_SWITCH(logn, return jr_pass_launch<L>(a, s));
   return hipSuccess;
 }
 

@@ -391,6 +391,25 @@ int same_rect(gt_ctx* c, Mirror& a, Mirror& b) {
   return GT_OK;
 }
 
+// src/forcing.f90 on a compact field: the phases of this call (host LCG)
+// go through a ring of pinned slots to the device.
+int forcing_on(gt_ctx* c, RectTables* t, double2* z, int32_t* seed) {
+  if (c->nf == 0) return GT_OK;
+  const unsigned slot = c->ph_next++ % kRing;
+  if (c->ph_ev_used[slot]) HIPCHK(hipEventSynchronize(c->ph_ev[slot]));
+  double2* hp = c->ph_host + (size_t)slot * c->nf;
+  double2* dp = c->ph_dev + (size_t)slot * c->nf;
+  forcing_phases(c->nf, seed, hp);
+  HIPCHK(hipMemcpyAsync(dp, hp, sizeof(double2) * c->nf, hipMemcpyHostToDevice, c->stream));
+  {
+    Timer tt(c, GT_K_FORCING);
+    HIPCHK(gt::launch_forcing(c->ph, c->nf, t->modes, dp, c->sdt * c->famp, z, c->stream));
+  }
+  HIPCHK(hipEventRecord(c->ph_ev[slot], c->stream));
+  c->ph_ev_used[slot] = true;
+  return GT_OK;
+}
+
 }  // namespace gtr
 
 // ===========================================================================
@@ -509,6 +528,7 @@ int gt_create(int32_t n, const double* p, gt_ctx** out) {
 int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  adj_free(c);
   for (auto& kv : c->present) {
     Mirror& m = kv.second;
     if (m.owned && m.dref) hipFree(m.dref);
@@ -856,19 +876,7 @@ int gt_forcing(gt_ctx* c, double* z, int32_t* seed) {
   CHK(ensure_cmp(c, *m));
   RectTables* t;
   CHK(rect_tables(c, m->rect, &t));
-  const unsigned slot = c->ph_next++ % kRing;
-  if (c->ph_ev_used[slot]) HIPCHK(hipEventSynchronize(c->ph_ev[slot]));
-  double2* hp = c->ph_host + (size_t)slot * c->nf;
-  double2* dp = c->ph_dev + (size_t)slot * c->nf;
-  forcing_phases(c->nf, seed, hp);
-  HIPCHK(hipMemcpyAsync(dp, hp, sizeof(double2) * c->nf, hipMemcpyHostToDevice, c->stream));
-  {
-    Timer tt(c, GT_K_FORCING);
-    HIPCHK(gt::launch_forcing(c->ph, c->nf, t->modes, dp, c->sdt * c->famp, m->dcmp,
-                              c->stream));
-  }
-  HIPCHK(hipEventRecord(c->ph_ev[slot], c->stream));
-  c->ph_ev_used[slot] = true;
+  CHK(forcing_on(c, t, m->dcmp, seed));
   set_cmp_only(*m);
   return GT_OK;
 }

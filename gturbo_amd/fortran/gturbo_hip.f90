@@ -13,6 +13,7 @@ module gturbo_hip
   public :: gt_synchronize
   public :: gt_fft_dir, gt_fft_inv, gt_produ, gt_trunc, gt_nlt, gt_step, gt_step4
   public :: gt_forcing, gt_spectrum, gt_fluxes, gt_glob
+  public :: gt_adj_forward, gt_adj_backward
   public :: GT_PLAN_D2Z, GT_PLAN_Z2D
 
   integer(c_int32_t), parameter :: GT_PLAN_D2Z = 1, GT_PLAN_Z2D = 2
@@ -140,6 +141,19 @@ module gturbo_hip
       type(c_ptr), value :: ctx
       real(c_double) :: z(*)
       real(c_double), intent(out) :: out(*)
+    end function
+    integer(c_int) function gt_adj_forward(ctx, w0, wt, nsteps, kchk, seed) &
+        bind(C, name="gt_adj_forward")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: w0(*), wt(*)
+      integer(c_int32_t), value :: nsteps, kchk
+      integer(c_int32_t), intent(inout) :: seed
+    end function
+    integer(c_int) function gt_adj_backward(ctx, gt, g0) bind(C, name="gt_adj_backward")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double) :: gt(*), g0(*)
     end function
   end interface
 

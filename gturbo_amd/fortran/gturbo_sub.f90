@@ -232,3 +232,27 @@ subroutine show_mem(fre, tot, step)
     real(tot) / 2.0**30
   call flush(6)
 end subroutine show_mem
+
+! The adjoint (no reference counterpart): the final field wT of nsteps steps
+! (step/step4, forcing) from the physical field w0, with a checkpoint every
+! kchk steps kept for adjoint_backward.  Arrays must be present.
+subroutine adjoint_forward(w0, wt, nsteps, kchk, seed)
+  use iso_c_binding
+  use gturbo_hip
+  implicit none
+  real(8) :: w0(*), wt(*)
+  integer :: nsteps, kchk, seed
+  integer(c_int32_t) :: s
+  s = int(seed, c_int32_t)
+  call gtcheck(gt_adj_forward(gtctx, w0, wt, int(nsteps, c_int32_t), int(kchk, c_int32_t), &
+                              s), 'adjoint_forward')
+  seed = int(s)
+end subroutine adjoint_forward
+
+! g0 = dL/dw0 from gt = dL/dwT for the run of the last adjoint_forward.
+subroutine adjoint_backward(gt, g0)
+  use gturbo_hip
+  implicit none
+  real(8) :: gt(*), g0(*)
+  call gtcheck(gt_adj_backward(gtctx, gt, g0), 'adjoint_backward')
+end subroutine adjoint_backward

@@ -146,6 +146,26 @@ int gt_timing_enable(gt_ctx *ctx, int32_t on);
  * and launch counts; resets the sums. */
 int gt_timing_read(gt_ctx *ctx, double *ms, int32_t *count);
 
+/* --- adjoint -------------------------------------------------------------
+ * Gradients with respect to the initial field, for initial-condition
+ * inversion.  The reference has no adjoint: these entries differentiate the
+ * run of src/main.f90:89-136 -- fft_dir and trunc of the input frame, then
+ * nsteps x (step or step4, forcing), then fft_inv -- through hand-written
+ * transposes of src/nlt.f90 (transposed transforms, adjoint products and
+ * alpha operator) and of the RK stages of src/rkstep.f90 (adjoint
+ * integrating factors).  Needs the 2/3 rule (truncate = 1).  Fields are
+ * present arrays of (N+2)*N doubles in physical space. */
+
+/* Runs the forward model from w0 for nsteps steps (forcing from *seed, which
+ * advances), writes the final field to wT and keeps a checkpoint of the
+ * state every kchk steps on the device for gt_adj_backward. */
+int gt_adj_forward(gt_ctx *ctx, const double *w0, double *wT, int32_t nsteps,
+                   int32_t kchk, int32_t *seed);
+/* Given gT = dL/dwT, the gradient g0 = dL/dw0 for the run of the last
+ * gt_adj_forward: each checkpoint segment is recomputed with its stage
+ * states kept, then stepped backwards. */
+int gt_adj_backward(gt_ctx *ctx, const double *gT, double *g0);
+
 /* --- batched ensembles ---------------------------------------------------
  * The reference runs an ensemble as copies of one run directory, each
  * submitted as its own job (src/reset.sh:12-77, src/jobscript:14), each job

@@ -51,6 +51,9 @@ def lib():
         L.or_step.argtypes = [C.c_void_p, _D, _D, _D, _D]
         L.or_step4.argtypes = [C.c_void_p, _D, _D, _D, _D, _D]
         L.or_forcing.argtypes = [C.c_void_p, _D, C.POINTER(C.c_int32)]
+        L.or_nlt_tl.argtypes = [C.c_void_p, _D, _D, _D]
+        L.or_step4_tl.argtypes = [C.c_void_p, _D, _D]
+        L.or_step_tl.argtypes = [C.c_void_p, _D, _D]
         L.or_forcing_phases.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int, _D]
         L.or_spectrum.argtypes = [C.c_void_p, _D, _D, _D]
         L.or_fluxes.argtypes = [C.c_void_p, _D, _D, _D, _D]
@@ -154,6 +157,20 @@ class Oracle:
         lib().or_step4(self.h, z, f() if znl is None else znl,
                        f() if znew is None else znew,
                        f() if u is None else u, f() if v is None else v)
+
+    def nlt_tl(self, z, dz):
+        """N'(z) dz (tangent of nlt at z)."""
+        out = self.field()
+        lib().or_nlt_tl(self.h, z, dz, out)
+        return out
+
+    def step_tl(self, z, dz):
+        """One step of z (step or step4 by the RK order) and of its tangent
+        dz, both in place."""
+        if self.rko == 4:
+            lib().or_step4_tl(self.h, z, dz)
+        else:
+            lib().or_step_tl(self.h, z, dz)
 
     def forcing(self, z, seed):
         s = C.c_int32(int(seed))

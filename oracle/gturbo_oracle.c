@@ -532,6 +532,160 @@ void or_step4(const or_ctx *c, double *z, double *znl, double *znew,
 }
 
 /* ------------------------------------------------------------------ */
+/* Tangent-linear model of the step (the reference has none): the exact  */
+/* linearisation of or_nlt / or_step / or_step4, used to check the      */
+/* library's adjoint (a dot-product test) and pinned by finite          */
+/* differences of the functions above.                                  */
+/* ------------------------------------------------------------------ */
+
+/* The spectral velocity of src/nlt.f90:22-36: u = i ky z/ka, v = -i kx z/ka. */
+static void velocity(const or_ctx *c, const double *z, double *u, double *v) {
+  const int n = c->n, nh = c->nh;
+  const size_t ns = ((size_t)n + 2) * n;
+  const double e = c->alpv / 2.0;
+  memset(u, 0, sizeof(double) * ns);
+  memset(v, 0, sizeof(double) * ns);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < nh; i++) {
+      if (i == 0 && j == 0) continue;
+      double k2 = c->fkxs[i] + c->fkys[j];
+      double ka = pow(k2, e);
+      double kx = c->fkx[i] / ka;
+      double ky = c->fky[j] / ka;
+      size_t p = 2 * (i + (size_t)nh * j);
+      u[p] = -ky * z[p + 1];
+      u[p + 1] = +ky * z[p];
+      v[p] = +kx * z[p + 1];
+      v[p + 1] = -kx * z[p];
+    }
+}
+
+/* out = N'(z) dz = -ik.(dq u + q du)^, the derivative of or_nlt at z. */
+void or_nlt_tl(const or_ctx *c, const double *z, const double *dz, double *out) {
+  const int n = c->n, nh = c->nh;
+  const size_t ns = ((size_t)n + 2) * n;
+  double *q = malloc(sizeof(double) * ns), *u = malloc(sizeof(double) * ns);
+  double *v = malloc(sizeof(double) * ns), *dq = malloc(sizeof(double) * ns);
+  double *du = malloc(sizeof(double) * ns), *dv = malloc(sizeof(double) * ns);
+  memcpy(q, z, sizeof(double) * ns);
+  memcpy(dq, dz, sizeof(double) * ns);
+  velocity(c, z, u, v);
+  velocity(c, dz, du, dv);
+  double *f[6] = {q, u, v, dq, du, dv};
+  for (int k = 0; k < 6; k++) or_fft_inv(c, f[k]);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n + 2; i++) {
+      size_t p = (size_t)j * (n + 2) + i;
+      double a = 0.0, b = 0.0;
+      if (i < n) {
+        a = dq[p] * u[p] + q[p] * du[p];
+        b = dq[p] * v[p] + q[p] * dv[p];
+      }
+      u[p] = a;
+      v[p] = b;
+    }
+  or_fft_dir(c, u);
+  or_fft_dir(c, v);
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < nh; i++) {
+      size_t p = 2 * (i + (size_t)nh * j);
+      out[p] = +c->fkx[i] * u[p + 1] + c->fky[j] * v[p + 1];
+      out[p + 1] = -c->fkx[i] * u[p] - c->fky[j] * v[p];
+    }
+  if (c->truncate == 1) or_trunc(c, out);
+  for (int k = 0; k < 6; k++) free(f[k]);
+}
+
+/* N(s) into k (s kept) and N'(s) ds into dk. */
+static void nlt_pair(const or_ctx *c, const double *s, const double *ds, double *k,
+                     double *dk, double *u, double *v) {
+  const size_t ns = ((size_t)c->n + 2) * c->n;
+  memcpy(k, s, sizeof(double) * ns);
+  or_nlt(c, k, u, v);
+  or_nlt_tl(c, s, ds, dk);
+}
+
+/* One or_step4 of z together with its tangent dz (z as or_step4 leaves it). */
+void or_step4_tl(const or_ctx *c, double *z, double *dz) {
+  const size_t nc = (size_t)c->nh * c->n, ns = 2 * nc;
+  const double *E = c->emk;
+  const double dt = c->dt, dt2 = c->dt2, dt3 = c->dt3, dt6 = c->dt6;
+  double *s = malloc(sizeof(double) * ns), *ds = malloc(sizeof(double) * ns);
+  double *k = malloc(sizeof(double) * ns), *dk = malloc(sizeof(double) * ns);
+  double *zn = malloc(sizeof(double) * ns), *dzn = malloc(sizeof(double) * ns);
+  double *u = malloc(sizeof(double) * ns), *v = malloc(sizeof(double) * ns);
+  nlt_pair(c, z, dz, k, dk, u, v);
+  for (size_t q = 0; q < nc; q++)
+    for (int r = 0; r < 2; r++) {
+      size_t p = 2 * q + r;
+      zn[p] = E[q] * E[q] * (z[p] + dt6 * k[p]);
+      dzn[p] = E[q] * E[q] * (dz[p] + dt6 * dk[p]);
+      s[p] = E[q] * (z[p] + dt2 * k[p]);
+      ds[p] = E[q] * (dz[p] + dt2 * dk[p]);
+    }
+  nlt_pair(c, s, ds, k, dk, u, v);
+  for (size_t q = 0; q < nc; q++)
+    for (int r = 0; r < 2; r++) {
+      size_t p = 2 * q + r;
+      zn[p] = zn[p] + dt3 * E[q] * k[p];
+      dzn[p] = dzn[p] + dt3 * E[q] * dk[p];
+      s[p] = E[q] * z[p] + dt2 * k[p];
+      ds[p] = E[q] * dz[p] + dt2 * dk[p];
+    }
+  nlt_pair(c, s, ds, k, dk, u, v);
+  for (size_t q = 0; q < nc; q++)
+    for (int r = 0; r < 2; r++) {
+      size_t p = 2 * q + r;
+      zn[p] = zn[p] + dt3 * E[q] * k[p];
+      dzn[p] = dzn[p] + dt3 * E[q] * dk[p];
+      s[p] = E[q] * (E[q] * z[p] + dt * k[p]);
+      ds[p] = E[q] * (E[q] * dz[p] + dt * dk[p]);
+    }
+  nlt_pair(c, s, ds, k, dk, u, v);
+  for (size_t p = 0; p < ns; p++) {
+    z[p] = zn[p] + dt6 * k[p];
+    dz[p] = dzn[p] + dt6 * dk[p];
+  }
+  z[0] = z[1] = dz[0] = dz[1] = 0.0;
+  free(s); free(ds); free(k); free(dk); free(zn); free(dzn); free(u); free(v);
+}
+
+/* One or_step of z together with its tangent dz. */
+void or_step_tl(const or_ctx *c, double *z, double *dz) {
+  const int n = c->n, nh = c->nh;
+  const size_t nc = (size_t)nh * n, ns = 2 * nc;
+  double *s = malloc(sizeof(double) * ns), *ds = malloc(sizeof(double) * ns);
+  double *k = malloc(sizeof(double) * ns), *dk = malloc(sizeof(double) * ns);
+  double *u = malloc(sizeof(double) * ns), *v = malloc(sizeof(double) * ns);
+  nlt_pair(c, z, dz, k, dk, u, v);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < nh; i++) {
+      size_t q = i + (size_t)nh * j, p = 2 * q;
+      double e = c->split ? c->emkx[i] * c->emky[j] : c->emk[q];
+      for (int r = 0; r < 2; r++) {
+        s[p + r] = e * (z[p + r] + c->dt2 * k[p + r]);
+        ds[p + r] = e * (dz[p + r] + c->dt2 * dk[p + r]);
+      }
+    }
+  s[0] = s[1] = ds[0] = ds[1] = 0.0;
+  nlt_pair(c, s, ds, k, dk, u, v);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < nh; i++) {
+      size_t q = i + (size_t)nh * j, p = 2 * q;
+      double e = c->split ? c->emkx[i] * c->emky[j] : c->emk[q];
+      for (int r = 0; r < 2; r++) {
+        z[p + r] = e * (e * z[p + r] + c->dt * k[p + r]);
+        dz[p + r] = e * (e * dz[p + r] + c->dt * dk[p + r]);
+      }
+    }
+  z[0] = z[1] = dz[0] = dz[1] = 0.0;
+  free(s); free(ds); free(k); free(dk); free(u); free(v);
+}
+
+/* ------------------------------------------------------------------ */
 /* Forcing: src/forcing.f90                                             */
 /* ------------------------------------------------------------------ */
 

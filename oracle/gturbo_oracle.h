@@ -56,6 +56,11 @@ void or_step(const or_ctx *c, double *z, double *znl, double *u, double *v);
 void or_step4(const or_ctx *c, double *z, double *znl, double *znew,
               double *u, double *v);
 void or_forcing(const or_ctx *c, double *z, int32_t *seed);
+/* Tangent-linear model (no reference counterpart; checks the adjoint):
+ * out = N'(z) dz, and one step of z with its tangent dz. */
+void or_nlt_tl(const or_ctx *c, const double *z, const double *dz, double *out);
+void or_step4_tl(const or_ctx *c, double *z, double *dz);
+void or_step_tl(const or_ctx *c, double *z, double *dz);
 /* Forcing phases for nsteps calls of forcing(): cs[2*(s*nf+k)+{0,1}] =
  * (cos te, sin te); seed advances as nsteps*nf draws of rann. */
 void or_forcing_phases(const or_ctx *c, int32_t *seed, int nsteps,

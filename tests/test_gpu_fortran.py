@@ -109,3 +109,34 @@ def test_fortran_driver_resume_chain(tmp_path):
     b = io.read_field(str(two / "fields" / "w.003"), n)
     assert io.read_seed(str(one / "files" / "seed.002")) == io.read_seed(str(two / "files" / "seed.003"))
     assert rel_l2(b, a) < 1e-5
+
+
+def test_fortran_initial_condition_inversion(tmp_path):
+    """gturbo_invert (Fortran, through adjoint_forward/adjoint_backward)
+    lowers the misfit of a perturbed first guess to an observation made by a
+    forward run from the true initial field."""
+    from gturbo_amd.adjoint import Adjoint
+    n = 64
+    p = Params(n=n, nu=1e-12, alpnu=4.0, mu=0.1, alpmu=0.0, alpv=1.0, dt=2e-3, rko=4,
+               npas=20, iout=10**6, imix=10**6, famp=0.0, k1f=3.0, k2f=5.0)
+    from gturbo_amd import ic
+    truth = ic.physical_field(p, InitialSpectrum(seed=21, p=2.0, k0=4.0, energy=0.5))
+    other = ic.physical_field(p, InitialSpectrum(seed=22, p=2.0, k0=4.0, energy=0.5))
+    guess = (truth + 0.2 * other).astype(np.float32)
+    with Adjoint(p, p.npas, kchk=5) as adj:
+        obs, _ = adj.forward(truth.astype(np.float64), 481929)
+    io.make_run_dir(str(tmp_path), p, guess, 481929, ifr=1, t=0.0)
+    io.write_field(str(tmp_path / "fields" / "obs.001"), obs.astype(np.float32))
+    exe = os.path.join(ROOT, "gturbo_amd", "fortran", "gturbo_invert")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (run __graft_entry__.build())")
+    r = subprocess.run([exe, str(n), "6"], cwd=str(tmp_path), capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-3000:])
+    assert r.returncode == 0, r.stderr
+    losses = [float(line.split()[3]) for line in r.stdout.splitlines()
+              if line.strip().startswith("iteration")]
+    assert len(losses) >= 4
+    assert all(b < a for a, b in zip(losses, losses[1:]))
+    assert losses[-1] < 0.5 * losses[0]
+    assert os.path.exists(tmp_path / "fields" / "winv.001")

@@ -416,3 +416,48 @@ def test_run_is_the_driver_sequence(O):
     o.fft_inv(b)
     assert seed_a == s and t == pytest.approx(3e-3)
     assert np.array_equal(a, b)
+
+
+# --- tangent-linear model (checks the library's adjoint) -------------------
+
+@pytest.mark.parametrize("alpv", [0.5, 1.0, 2.0])
+def test_nlt_tangent_is_the_exact_derivative(oracle_mod, alpv):
+    """N is quadratic in z, so a central difference of or_nlt equals
+    N'(z) d up to rounding, for any step size."""
+    n = 32
+    o = oracle_mod.Oracle(params_for(n, alpv=alpv))
+    z = random_spectral(n, 5)
+    d = random_spectral(n, 6)
+    tl = o.nlt_tl(z, d)
+    eps = 0.25
+    zp, zm = z + eps * d, z - eps * d
+    o.nlt(zp)
+    o.nlt(zm)
+    fd = (zp - zm) / (2 * eps)
+    assert rel_l2(tl, fd) < 1e-13
+    # linear in the direction
+    d2 = random_spectral(n, 7)
+    assert rel_l2(o.nlt_tl(z, 2.0 * d - 3.0 * d2),
+                  2.0 * tl - 3.0 * o.nlt_tl(z, d2)) < 1e-13
+
+
+@pytest.mark.parametrize("rko", [4, 2])
+def test_step_tangent_matches_finite_differences(oracle_mod, rko):
+    n = 32
+    p = params_for(n, alpv=1.0, rko=rko, famp=0.0)
+    o = oracle_mod.Oracle(p)
+    z = random_spectral(n, 8) * 4.0
+    d = random_spectral(n, 9)
+    zt, dt_ = z.copy(), d.copy()
+    o.step_tl(zt, dt_)
+    z4 = z.copy()
+    o.step4(z4) if rko == 4 else o.step(z4)
+    assert np.array_equal(zt, z4)  # the state advances as the plain step
+    errs = []
+    for eps in (1e-3, 1e-4):
+        zp, zm = z + eps * d, z - eps * d
+        for a in (zp, zm):
+            o.step4(a) if rko == 4 else o.step(a)
+        errs.append(rel_l2(dt_, (zp - zm) / (2 * eps)))
+    assert errs[1] < 1e-7
+    assert errs[1] < errs[0] * 0.05  # second-order convergence of the difference
