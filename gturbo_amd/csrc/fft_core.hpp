@@ -13,10 +13,11 @@
 // which lets the passes that carry a second field run at 4 waves per SIMD.
 //
 // Twiddles come from a table W[m] = exp(-2 pi i m / N), m < N, built on the
-// host in extended precision and rounded once.  A radix-8 butterfly loads
-// W^k, W^2k, W^4k and forms W^3k, W^5k, W^6k, W^7k by complex products (at
-// most two extra roundings), which keeps three twiddles live instead of
-// seven.
+// host in extended precision and rounded once.  A butterfly loads only W^k
+// and forms W^2k ... W^7k by complex products (at most four extra
+// roundings): on gfx950 these passes are bound by memory-instruction
+// latency, not by FP64 arithmetic, and one load per butterfly measured
+// faster than three (and three faster than seven).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -144,11 +145,11 @@ __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
       double2 w[R];
       w[1] = tw[k * ST];
       if constexpr (R >= 4) {
-        w[2] = tw[2 * k * ST];
+        w[2] = cmul(w[1], w[1]);
         w[3] = cmul(w[1], w[2]);
       }
       if constexpr (R == 8) {
-        w[4] = tw[4 * k * ST];
+        w[4] = cmul(w[2], w[2]);
         w[5] = cmul(w[1], w[4]);
         w[6] = cmul(w[2], w[4]);
         w[7] = cmul(w[3], w[4]);
