@@ -120,11 +120,39 @@ struct Line {
   static constexpr int RLAST = 1 << (LOGN % 3);  // trailing radix (1 = none)
 };
 
+// Entries of the twiddle table the passes of a line read: W^(k Ns') with
+// k Ns' < N/8 for the radix-8 passes and < N/R for a trailing radix-R pass.
+template <int LOGN>
+constexpr int tw_span() {
+  return (1 << LOGN) / (Line<LOGN>::RLAST > 1 ? Line<LOGN>::RLAST : 8);
+}
+// Up to N = 4096 the kernels keep that part of the table in LDS (at most
+// 512 entries, beside their line buffers); larger lines read it from memory.
+template <int LOGN>
+constexpr bool tw_in_lds() { return LOGN <= 12; }
+template <int LOGN>
+constexpr int tw_lds_size() { return tw_in_lds<LOGN>() ? tw_span<LOGN>() : 1; }
+
+// Copies the table's first tw_span entries to LDS (every thread of the
+// workgroup calls it; ends with a barrier) and returns the table the
+// transforms should read.
+template <int LOGN>
+__device__ __forceinline__ const double2* stage_tw(double2* twl, const double2* __restrict__ tw,
+                                                   int tid, int nthr) {
+  if constexpr (tw_in_lds<LOGN>()) {
+    for (int i = tid; i < tw_span<LOGN>(); i += nthr) twl[i] = tw[i];
+    __syncthreads();
+    return twl;
+  } else {
+    return tw;
+  }
+}
+
 // One Stockham pass: radix R, stride Ns.  Butterfly jj (< N/R) reads
 // x[jj + (N/R) r'] and writes y[(jj/Ns) Ns R + jj%Ns + Ns r'].
 template <int LOGN, int S, int R, int NS, bool LAST>
 __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
-                                         const double2* __restrict__ tw) {
+                                         const double2* tw) {
   constexpr int N = 1 << LOGN;
   constexpr int TPL = N / kE;
   constexpr int NB = kE / R;  // butterflies per thread
@@ -183,7 +211,7 @@ __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
 // round (same result up to rounding; the faster one depends on N).
 template <int LOGN, int S, int SCHED = 0>
 __device__ __forceinline__ void fft_reg(double2* a, double2* buf, int j,
-                                        const double2* __restrict__ tw) {
+                                        const double2* tw) {
   using L = Line<LOGN>;
   constexpr int NP = L::NPASS8 + (L::RLAST > 1 ? 1 : 0);
   if constexpr (SCHED == 0 || L::RLAST == 1) {

@@ -118,7 +118,7 @@ __device__ __forceinline__ void line_map(int tid, int& line, int& j) {
 // order of the transforms (fft_reg).
 template <int LOGN, int MODE, int AK, int TR, int BT, int SCHED>
 __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
-                                       double2* lds) {
+                                       double2* lds, const double2* tw) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
   constexpr int LINE = Line<LOGN>::PADDED;
   constexpr bool FWD = MODE != C_START;
@@ -142,13 +142,13 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
     const double2* srcA = a.y2 + (size_t)tile * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcA[(unsigned)((j + TPL * r) * BT + b)];
-    fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);
+    fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
 #pragma unroll
     for (int r = 0; r < kE; ++r) A[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
     const double2* srcB = a.y2 + (size_t)(a.ntiles + tile) * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcB[(unsigned)((j + TPL * r) * BT + b)];
-    fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);
+    fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
     int je = j;
     // The epilogue's indices (and so its loads) depend on the transform's
     // output: nothing of the epilogue is scheduled inside the last pass.
@@ -274,7 +274,7 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
         }
         z[r] = v;
       }
-      fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
+      fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
       double2* dst = a.y1 + (size_t)(f * a.ntiles + tile) * N * BT;
 #pragma unroll
       for (int r = 0; r < kE; ++r) dst[(unsigned)((j + TPL * r) * BT + b)] = z[r];
@@ -287,7 +287,9 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
     c_pass(PassArgs a) {
   constexpr int BT = bt_of(LOGN);
   __shared__ double2 lds[BT * Line<LOGN>::PADDED];
-  c_body<LOGN, MODE, AK, TR, BT, 0>(a, blockIdx.x, threadIdx.x, lds);
+  __shared__ double2 twl[tw_lds_size<LOGN>()];
+  const double2* tw = stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
+  c_body<LOGN, MODE, AK, TR, BT, 0>(a, blockIdx.x, threadIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -358,7 +360,7 @@ __device__ __forceinline__ void split_store(const double2* w, double2* buf, int 
 // two-column tiles a wave's access covers whole 64-byte runs of a tile.
 template <int LOGN, int TR, int BT, int NL, int SCHED>
 __device__ __forceinline__ void r_body(const PassArgs& a, int g, int tid,
-                                       double2* lds) {
+                                       double2* lds, const double2* tw) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
   constexpr int LINE = Line<LOGN>::PADDED;
   int pl, j;
@@ -372,14 +374,14 @@ __device__ __forceinline__ void r_body(const PassArgs& a, int g, int tid,
   double2 z[kE];
   double qv[kE];
   build_two<LOGN, TR, BT>(z, j, a.y1, offQ, offV, kxm);
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) qv[r] = z[r].x * z[r].y;  // produ.f90:16
   build_two<LOGN, TR, BT>(z, j, a.y1, offQ, offU, kxm);
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(z[r].x * z[r].y, qv[r]);  // produ.f90:15
-  fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);
+  fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, offQ, offQ + fstride, kxm);
 }
 
@@ -390,7 +392,7 @@ __device__ __forceinline__ void r_body(const PassArgs& a, int g, int tid,
 // transforms per row pair, only q carried across them.
 template <int LOGN, int TR, int BT, int NL, int SCHED>
 __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
-                                          double2* lds) {
+                                          double2* lds, const double2* tw) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
   constexpr int LINE = Line<LOGN>::PADDED;
   int pl, j;
@@ -404,20 +406,20 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
 
   double2 z[kE], qq[kE];
   build_two<LOGN, TR, BT>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) qq[r] = z[r];                   // q0 + i q1
   build_two<LOGN, TR, BT>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);                   // (q v)0 + i (q v)1
+  fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);                   // (q v)0 + i (q v)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, fstride + r0, fstride + r1, kxm);
   build_two<LOGN, TR, BT>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1, SCHED>(z, buf, j, a.tw);                   // (q u)0 + i (q u)1
+  fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);                   // (q u)0 + i (q u)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, r0, r1, kxm);
 }
 
@@ -426,16 +428,21 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
     r_pass(PassArgs a) {
   constexpr int RP = rows_of(LOGN);
   __shared__ double2 lds[RP * Line<LOGN>::PADDED];
-  r_body<LOGN, TR, bt_of(LOGN), RP, 0>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x, lds);
+  __shared__ double2 twl[tw_lds_size<LOGN>()];
+  const double2* tw = stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
+  r_body<LOGN, TR, bt_of(LOGN), RP, 0>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x, lds,
+                                       tw);
 }
 
 // One row pair per workgroup: two workgroups share a CU.
 template <int LOGN, int TR>
 __global__ void __launch_bounds__((1 << LOGN) / kE)
     r_pass_pair(PassArgs a) {
+  // Twiddles from memory: these workgroups are short-lived (one row pair),
+  // and staging the table in LDS measured slower here.
   __shared__ double2 lds[Line<LOGN>::PADDED];
   pair_body<LOGN, TR, bt_of(LOGN), 1, 0>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x,
-                                         lds);
+                                         lds, a.tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -576,6 +583,8 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
   constexpr bool FWD = MODE != J_START4 && MODE != J_START2;
   constexpr bool INV = MODE != J_END;
   __shared__ double2 lds[BT * LINE];
+  __shared__ double2 twl[tw_lds_size<LOGN>()];
+  const double2* tw = stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
   int b, j;
   line_map<TPL, BT>(threadIdx.x, b, j);
   const int tile = blockIdx.x;
@@ -596,12 +605,12 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
     const size_t fs = (size_t)a.ntiles * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = y2[fs + (unsigned)((j + TPL * r) * BT + b)];
-    fft_reg<LOGN, -1>(z, buf, j, a.tw);  // F(q lambda_x)
+    fft_reg<LOGN, -1>(z, buf, j, tw);  // F(q lambda_x)
 #pragma unroll
     for (int r = 0; r < kE; ++r) C[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = y2[2 * fs + (unsigned)((j + TPL * r) * BT + b)];
-    fft_reg<LOGN, -1>(z, buf, j, a.tw);  // F(q lambda_y)
+    fft_reg<LOGN, -1>(z, buf, j, tw);  // F(q lambda_y)
     int jc = j;
     asm volatile("" : "+v"(jc) : "v"(z[0].x), "v"(z[kE - 1].y));
 #pragma unroll
@@ -619,7 +628,7 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
     }
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = y2[(unsigned)((j + TPL * r) * BT + b)];
-    fft_reg<LOGN, -1>(z, buf, j, a.tw);  // F(u lambda_x + v lambda_y)
+    fft_reg<LOGN, -1>(z, buf, j, tw);  // F(u lambda_x + v lambda_y)
     int je = j;
     asm volatile("" : "+v"(je) : "v"(z[0].x), "v"(z[kE - 1].y));
 #pragma unroll
@@ -717,7 +726,7 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
         }
         z[r] = v;
       }
-      fft_reg<LOGN, +1>(z, buf, j, a.tw);
+      fft_reg<LOGN, +1>(z, buf, j, tw);
       double2* dst = a.y1 + (size_t)(f * a.ntiles + tile) * N * BT;
 #pragma unroll
       for (int r = 0; r < kE; ++r) dst[(unsigned)((j + TPL * r) * BT + b)] = z[r];
@@ -754,6 +763,8 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
   constexpr int N = 1 << LOGN, TPL = N / kE, RP = rows_of(LOGN), BT = bt_of(LOGN);
   constexpr int LINE = Line<LOGN>::PADDED;
   __shared__ double2 lds[RP * LINE];
+  __shared__ double2 twl[tw_lds_size<LOGN>()];
+  const double2* tw = stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
   int pl, j;
   line_map<TPL, RP>(threadIdx.x, pl, j);
   const int yrow = xcd_remap(blockIdx.x, gridDim.x) * RP + pl;
@@ -765,23 +776,23 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
   double2 z[kE], l[kE];
   double ap[kE];
   build_two<LOGN, TR, BT>(z, j, a.y1, off, off + fs, kxm);  // LX + i LY
-  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) l[r] = z[r];                   // lambda_x + i lambda_y
   build_two<LOGN, TR, BT>(z, j, a.y1, off + 2 * fs, off + 3 * fs, kxm);  // Q + i U
-  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) {
     ap[r] = z[r].y * l[r].x;                                  // u lambda_x
     z[r] = make_double2(z[r].x * l[r].x, z[r].x * l[r].y);    // q lambda_x + i q lambda_y
   }
-  fft_reg<LOGN, -1>(z, buf, j, a.tw);
+  fft_reg<LOGN, -1>(z, buf, j, tw);
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, off + fs, off + 2 * fs, kxm);
   build_one<LOGN, TR, BT>(z, j, a.y1, off + 4 * fs, kxm);   // V
-  fft_reg<LOGN, +1>(z, buf, j, a.tw);
+  fft_reg<LOGN, +1>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(ap[r] + z[r].x * l[r].y, 0.0);
-  fft_reg<LOGN, -1>(z, buf, j, a.tw);                          // F(u lambda_x + v lambda_y)
+  fft_reg<LOGN, -1>(z, buf, j, tw);                          // F(u lambda_x + v lambda_y)
 #pragma unroll
   for (int r = 0; r <= kE / 2; ++r) {
     const int k = j + TPL * r;
@@ -932,10 +943,27 @@ template hipError_t ref_fft_launch<GT_INST_LOGN>(int, double*, const double2*, h
 // -DGT_ENS_SCHED=SCHED.
 // ---------------------------------------------------------------------------
 
+// Twiddles of the item's size in LDS: staged again only when the member's
+// table changes (barriers on both sides: the previous item's last pass may
+// still read the old table).
+__device__ __forceinline__ const double2* ens_stage_tw(double2* twl, const PassArgs& a,
+                                                       const double2*& cur) {
+  if (a.tw != cur) {
+    __syncthreads();
+    const int n = a.ph.n;
+    const int span = n == 128 ? tw_span<7>() : n == 256 ? tw_span<8>()
+                   : n == 512 ? tw_span<9>() : n == 1024 ? tw_span<10>() : tw_span<11>();
+    for (int i = threadIdx.x; i < span; i += blockDim.x) twl[i] = a.tw[i];
+    __syncthreads();
+    cur = a.tw;
+  }
+  return twl;
+}
+
 template <int T, int MODE, int SCHED, int L>
 __device__ __forceinline__ void ens_c_one(const PassArgs& a, int tile, int tid,
-                                          double2* lds) {
-  c_body<L, MODE, 2, 1, ens_lines(T, L), SCHED>(a, tile, tid, lds);
+                                          double2* lds, const double2* tw) {
+  c_body<L, MODE, 2, 1, ens_lines(T, L), SCHED>(a, tile, tid, lds, tw);
 }
 
 template <int T, int MODE, int SCHED>
@@ -943,30 +971,34 @@ __global__ void __launch_bounds__(T, 4)
     ens_c_pass(const PassArgs* __restrict__ descs, const uint32_t* __restrict__ items,
                int nitems) {
   __shared__ double2 lds[T * 17 / 2];
+  __shared__ double2 twl[tw_span<kEnsMaxLogn>()];
+  const double2* cur = nullptr;
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const uint32_t it = items[w];
     const PassArgs& a = descs[it >> 16];
     const int tile = (int)(it & 0xFFFFu);
+    const double2* tw = ens_stage_tw(twl, a, cur);
     // Opaque per item: nothing derived from the thread index is hoisted out
     // of the loop (and kept live over every size's code).
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     switch (a.ph.n) {
-      case 128: ens_c_one<T, MODE, SCHED, 7>(a, tile, tid, lds); break;
-      case 256: ens_c_one<T, MODE, SCHED, 8>(a, tile, tid, lds); break;
-      case 512: ens_c_one<T, MODE, SCHED, 9>(a, tile, tid, lds); break;
-      case 1024: ens_c_one<T, MODE, SCHED, 10>(a, tile, tid, lds); break;
-      case 2048: ens_c_one<T, MODE, SCHED, 11>(a, tile, tid, lds); break;
+      case 128: ens_c_one<T, MODE, SCHED, 7>(a, tile, tid, lds, tw); break;
+      case 256: ens_c_one<T, MODE, SCHED, 8>(a, tile, tid, lds, tw); break;
+      case 512: ens_c_one<T, MODE, SCHED, 9>(a, tile, tid, lds, tw); break;
+      case 1024: ens_c_one<T, MODE, SCHED, 10>(a, tile, tid, lds, tw); break;
+      case 2048: ens_c_one<T, MODE, SCHED, 11>(a, tile, tid, lds, tw); break;
       default: break;
     }
   }
 }
 
 template <int TX, int TY, int SCHED, int L>
-__device__ __forceinline__ void ens_r_one(const PassArgs& a, int g, int tid, double2* lds) {
+__device__ __forceinline__ void ens_r_one(const PassArgs& a, int g, int tid, double2* lds,
+                                          const double2* tw) {
   constexpr int NL = ens_lines(TX, L), BT = ens_lines(TY, L);
-  if (a.xvar) pair_body<L, 1, BT, NL, SCHED>(a, g, tid, lds);
-  else r_body<L, 1, BT, NL, SCHED>(a, g, tid, lds);
+  if (a.xvar) pair_body<L, 1, BT, NL, SCHED>(a, g, tid, lds, tw);
+  else r_body<L, 1, BT, NL, SCHED>(a, g, tid, lds, tw);
 }
 
 template <int TX, int TY, int SCHED>
@@ -974,18 +1006,21 @@ __global__ void __launch_bounds__(TX, 4)
     ens_r_pass(const PassArgs* __restrict__ descs, const uint32_t* __restrict__ items,
                int nitems) {
   __shared__ double2 lds[TX * 17 / 2];
+  __shared__ double2 twl[tw_span<kEnsMaxLogn>()];
+  const double2* cur = nullptr;
   for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
     const uint32_t it = items[w];
     const PassArgs& a = descs[it >> 16];
     const int g = (int)(it & 0xFFFFu);
+    const double2* tw = ens_stage_tw(twl, a, cur);
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     switch (a.ph.n) {
-      case 128: ens_r_one<TX, TY, SCHED, 7>(a, g, tid, lds); break;
-      case 256: ens_r_one<TX, TY, SCHED, 8>(a, g, tid, lds); break;
-      case 512: ens_r_one<TX, TY, SCHED, 9>(a, g, tid, lds); break;
-      case 1024: ens_r_one<TX, TY, SCHED, 10>(a, g, tid, lds); break;
-      case 2048: ens_r_one<TX, TY, SCHED, 11>(a, g, tid, lds); break;
+      case 128: ens_r_one<TX, TY, SCHED, 7>(a, g, tid, lds, tw); break;
+      case 256: ens_r_one<TX, TY, SCHED, 8>(a, g, tid, lds, tw); break;
+      case 512: ens_r_one<TX, TY, SCHED, 9>(a, g, tid, lds, tw); break;
+      case 1024: ens_r_one<TX, TY, SCHED, 10>(a, g, tid, lds, tw); break;
+      case 2048: ens_r_one<TX, TY, SCHED, 11>(a, g, tid, lds, tw); break;
       default: break;
     }
   }
