@@ -224,3 +224,52 @@ def test_run_euler256_matches_oracle(oracle_mod):
     err = rel_l2(zg[:, :p.n], zo[:, :p.n])
     print(f"euler256 1000-step rel L2 = {err:.3e}")
     assert err < 1e-10
+
+
+@pytest.mark.parametrize("rko", [4, 2])
+def test_headline_workload_steps_match_oracle(oracle_mod, rko):
+    """The 4096^2 headline workload (alpha=2, dt=5e-5, nu=1e-28 k^8, mu=0.02,
+    ring (63,65], seed-2 initial field) for 6 steps with forcing, at its full
+    size: vorticity within 1e-12 relative L2 of the oracle, seed bit-exact."""
+    from gturbo_amd import ic, io, workloads
+    w = workloads()[f"euler4096_rk{rko}"]
+    p = w.params
+    s, o = _pair(oracle_mod, p)
+    zg = io.field_to_ref(ic.physical_field(p, w.ic))
+    zo = zg.copy()
+    print("4096^2 RK%d, 6 steps" % rko, flush=True)
+    seed_g, _ = s.run(zg, w.forcing_seed, 6)
+    seed_o, _, _ = o.run(zo, w.forcing_seed, 6)
+    assert seed_g == seed_o
+    err = rel_l2(zg[:, :p.n], zo[:, :p.n])
+    print(f"euler4096 RK{rko} 6-step rel L2 = {err:.3e}")
+    assert err < 1e-12
+
+
+def test_sqg1024_workload_with_diagnostics_matches_oracle(oracle_mod):
+    """The 1024^2 SQG workload (alpha=1, RK4, dt=2e-4, nu=1e-22 k^8,
+    mu=0.05, ring (31,33], seed-1 initial field, spectra and fluxes every 100
+    steps) over its first 200 steps: vorticity within 1e-10, spectra, fluxes
+    and global sums within 1e-9 of the oracle."""
+    from gturbo_amd import ic, io, workloads
+    w = workloads()["sqg1024"]
+    p = w.params
+    s, o = _pair(oracle_mod, p)
+    zg = io.field_to_ref(ic.physical_field(p, w.ic))
+    zo = zg.copy()
+    got = []
+    print("sqg1024, 200 steps", flush=True)
+    seed_g, _ = s.run(zg, w.forcing_seed, 200, diag=lambda *d: got.append(d))
+    seed_o, _, diag = o.run(zo, w.forcing_seed, 200, with_diag=True)
+    assert seed_g == seed_o
+    err = rel_l2(zg[:, :p.n], zo[:, :p.n])
+    print(f"sqg1024 200-step rel L2 = {err:.3e}")
+    assert err < 1e-10
+    nb = p.n // 2
+    assert len(got) == diag.shape[0] == 2
+    for (ipas, sp, sz, fle, flz, g), row in zip(got, diag):
+        osp, osz, ofle, oflz = (row[i * nb:(i + 1) * nb] for i in range(4))
+        assert rel_l2(sp, osp) < 1e-9 and rel_l2(sz, osz) < 1e-9
+        assert np.abs(fle - ofle).max() <= 1e-9 * np.abs(ofle).max()
+        assert np.abs(flz - oflz).max() <= 1e-9 * np.abs(oflz).max()
+        assert np.allclose(g, row[4 * nb:], rtol=1e-9, atol=0)
