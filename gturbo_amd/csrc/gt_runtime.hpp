@@ -3,6 +3,7 @@
 // table entries and the helpers that act on them.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <string>
 #include <unordered_map>
@@ -129,6 +130,16 @@ int r_pass(gt_ctx* c, const gt::PassArgs& a);
 void forcing_phases(int nf, int32_t* seed, double2* out);
 int forcing_on(gt_ctx* c, RectTables* t, double2* z, int32_t* seed);
 void adj_free(gt_ctx* c);  // gturbo_adjoint.cpp
+
+// roctx range over one C entry point (rocprofv3 --marker-trace shows them;
+// without a profiler attached they cost a call each).
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
+#define GT_RANGE() gtr::Range gt_range_(__func__)
 
 // Timer of one launch class on the context's stream (GT_K_* of gturbo.h).
 struct Timer {

@@ -391,6 +391,7 @@ int gt_ens_create(int32_t nmem, const int32_t* n, const double* params, gt_ens**
 }
 
 int gt_ens_set_field(gt_ens* e, int32_t m, const double* z, int32_t space) {
+  GT_RANGE();
   CHK(check_member(e, m));
   if (!z) return fail(GT_ERR_ARG, "null field");
   if (space != GT_SPACE_PHYSICAL && space != GT_SPACE_SPECTRAL)
@@ -418,6 +419,7 @@ int gt_ens_set_field(gt_ens* e, int32_t m, const double* z, int32_t space) {
 }
 
 int gt_ens_get_field(gt_ens* e, int32_t m, double* z, int32_t space) {
+  GT_RANGE();
   CHK(check_member(e, m));
   if (!z) return fail(GT_ERR_ARG, "null field");
   if (space != GT_SPACE_PHYSICAL && space != GT_SPACE_SPECTRAL)
@@ -460,6 +462,7 @@ int gt_ens_schedule(gt_ens* e, int32_t* cfg, int32_t set) {
 }
 
 int gt_ens_run(gt_ens* e, int32_t nsteps) {
+  GT_RANGE();
   if (!e) return fail(GT_ERR_ARG, "null ensemble");
   if (nsteps < 0) return fail(GT_ERR_ARG, "negative step count");
   if (nsteps == 0) return GT_OK;
@@ -493,6 +496,7 @@ int gt_ens_synchronize(gt_ens* e) {
 }
 
 int gt_ens_tune(gt_ens* e, int32_t steps, int32_t* cfg_out, double* ms_out) {
+  GT_RANGE();
   if (!e) return fail(GT_ERR_ARG, "null ensemble");
   if (steps < 1) return fail(GT_ERR_ARG, "steps must be positive");
   HIPCHK(hipStreamSynchronize(e->stream));

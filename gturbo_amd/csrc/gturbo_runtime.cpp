@@ -617,6 +617,7 @@ int gt_mem_info(size_t* fr, size_t* tot) {
 }
 
 int gt_enter_data(gt_ctx* c, void* host, int64_t bytes, int32_t copyin) {
+  GT_RANGE();
   if (!c || !host || bytes <= 0) return fail(GT_ERR_ARG, "bad enter_data arguments");
   if (c->present.count(host)) return fail(GT_ERR_ARG, "array %p already present", host);
   Mirror m;
@@ -667,6 +668,7 @@ int gt_unmap_data(gt_ctx* c, void* key) {
 }
 
 int gt_exit_data(gt_ctx* c, void* host, int32_t copyout) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   auto it = c->present.find(host);
   if (it == c->present.end()) return fail(GT_ERR_NOT_PRESENT, "array %p not present", host);
@@ -680,6 +682,7 @@ int gt_exit_data(gt_ctx* c, void* host, int32_t copyout) {
 }
 
 int gt_update_host(gt_ctx* c, void* host) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   auto it = c->present.find(host);
   if (it == c->present.end()) return fail(GT_ERR_NOT_PRESENT, "array %p not present", host);
@@ -695,6 +698,7 @@ int gt_update_host(gt_ctx* c, void* host) {
 int gt_sync_ref(gt_ctx* c, void* key) { return gt_update_host(c, key); }
 
 int gt_update_device(gt_ctx* c, void* host) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   auto it = c->present.find(host);
   if (it == c->present.end()) return fail(GT_ERR_NOT_PRESENT, "array %p not present", host);
@@ -708,6 +712,7 @@ int gt_update_device(gt_ctx* c, void* host) {
 }
 
 int gt_copy(gt_ctx* c, void* dst, const void* src) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   Mirror *d, *s;
   CHK(lookup(c, dst, &d));
@@ -742,6 +747,7 @@ int gt_synchronize(gt_ctx* c) {
 // ---- transforms and operators ----------------------------------------------
 
 int gt_fft_dir(gt_ctx* c, double* z, int32_t plan) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   if (plan != GT_PLAN_D2Z) return fail(GT_ERR_ARG, "fft_dir needs a D2Z plan (got %d)", plan);
   Mirror* m;
@@ -754,6 +760,7 @@ int gt_fft_dir(gt_ctx* c, double* z, int32_t plan) {
 }
 
 int gt_fft_inv(gt_ctx* c, double* z, int32_t plan) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   if (plan != GT_PLAN_Z2D) return fail(GT_ERR_ARG, "fft_inv needs a Z2D plan (got %d)", plan);
   Mirror* m;
@@ -766,6 +773,7 @@ int gt_fft_inv(gt_ctx* c, double* z, int32_t plan) {
 }
 
 int gt_produ(gt_ctx* c, const double* z, double* u, double* v) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   Mirror *mz, *mu, *mv;
   CHK(lookup(c, z, &mz));
@@ -782,6 +790,7 @@ int gt_produ(gt_ctx* c, const double* z, double* u, double* v) {
 }
 
 int gt_trunc(gt_ctx* c, double* z) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   Mirror* m;
   CHK(lookup(c, z, &m));
@@ -801,6 +810,7 @@ int gt_trunc(gt_ctx* c, double* z) {
 }
 
 int gt_nlt(gt_ctx* c, double* z, double* u, double* v) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   Mirror *m, *mu, *mv;
   CHK(lookup(c, z, &m));
@@ -818,6 +828,7 @@ int gt_nlt(gt_ctx* c, double* z, double* u, double* v) {
 }
 
 int gt_step(gt_ctx* c, double* z, double* znl, double* u, double* v) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   Mirror *m, *m1, *mu, *mv;
   CHK(lookup(c, z, &m));
@@ -839,6 +850,7 @@ int gt_step(gt_ctx* c, double* z, double* znl, double* u, double* v) {
 }
 
 int gt_step4(gt_ctx* c, double* z, double* znl, double* znew, double* u, double* v) {
+  GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
   Mirror *m, *m1, *mn, *mu, *mv;
   CHK(lookup(c, z, &m));
@@ -869,6 +881,7 @@ int gt_step4(gt_ctx* c, double* z, double* znl, double* znew, double* u, double*
 }
 
 int gt_forcing(gt_ctx* c, double* z, int32_t* seed) {
+  GT_RANGE();
   if (!c || !seed) return fail(GT_ERR_ARG, "null argument");
   Mirror* m;
   CHK(lookup(c, z, &m));
@@ -884,6 +897,7 @@ int gt_forcing(gt_ctx* c, double* z, int32_t* seed) {
 // ---- diagnostics ------------------------------------------------------------
 
 int gt_spectrum(gt_ctx* c, const double* z, double* sp, double* sz) {
+  GT_RANGE();
   if (!c || !sp || !sz) return fail(GT_ERR_ARG, "null argument");
   Mirror* m;
   CHK(lookup(c, z, &m));
@@ -902,6 +916,7 @@ int gt_spectrum(gt_ctx* c, const double* z, double* sp, double* sz) {
 }
 
 int gt_fluxes(gt_ctx* c, const double* z, const double* znl, double* fle, double* flz) {
+  GT_RANGE();
   if (!c || !fle || !flz) return fail(GT_ERR_ARG, "null argument");
   Mirror *m, *mn;
   CHK(lookup(c, z, &m));
@@ -924,6 +939,7 @@ int gt_fluxes(gt_ctx* c, const double* z, const double* znl, double* fle, double
 }
 
 int gt_glob(gt_ctx* c, const double* z, double* out) {
+  GT_RANGE();
   if (!c || !out) return fail(GT_ERR_ARG, "null argument");
   Mirror* m;
   CHK(lookup(c, z, &m));

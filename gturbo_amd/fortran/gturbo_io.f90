@@ -62,3 +62,39 @@ subroutine write_w(u, nome)
   close(90)
   deallocate(w)
 end subroutine write_w
+
+! FP64 checkpoint (no reference counterpart): the spectral state z(NX+2,NY)
+! in double precision, fields/z.NNN, one unformatted record.  A run that
+! resumes from it continues bit for bit; the float32 frame w.NNN of
+! src/write.f90 rounds the state.
+subroutine writez(z, ifr)
+  use gturbo_state
+  implicit none
+  real(8) :: z(nx + 2, nx)
+  integer :: ifr
+  character(len=40) :: nome
+  write(nome, "('./fields/z.',i3.3)") ifr
+  open(unit=92, file=nome, form='unformatted', status='replace')
+  write(92) z
+  close(92)
+end subroutine writez
+
+logical function havez(ifr)
+  implicit none
+  integer :: ifr
+  character(len=40) :: nome
+  write(nome, "('./fields/z.',i3.3)") ifr
+  inquire(file=nome, exist=havez)
+end function havez
+
+subroutine readz(z, ifr)
+  use gturbo_state
+  implicit none
+  real(8) :: z(nx + 2, nx)
+  integer :: ifr
+  character(len=40) :: nome
+  write(nome, "('./fields/z.',i3.3)") ifr
+  open(unit=92, file=nome, form='unformatted', action='read', status='old')
+  read(92) z
+  close(92)
+end subroutine readz

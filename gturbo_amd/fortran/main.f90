@@ -1,12 +1,16 @@
 ! main.f90 -- the driver of src/main.f90:1-195 on the MI355X library.
 !
-!   gturbo_run [N]        (N = NX = NY, default 8192 as src/modules.f90:4)
+!   gturbo_run [N] [exact]   (N = NX = NY, default 8192 as src/modules.f90:4)
 !
 ! Reads curframe.dat, files/seed.NNN, params.dat and (when the frame is > 0)
 ! fields/w.NNN; runs npas steps with diagnostics every imix steps and
 ! snapshots every iout steps; writes fields/w.(NNN+1), files/seed.(NNN+1)
 ! and curframe.dat.  The OpenACC data region of the reference is the
 ! library's present table (gt_enter_data ... gt_exit_data).
+! It also writes the FP64 spectral state as fields/z.(NNN+1); with "exact" a
+! run starting at frame NNN resumes from fields/z.NNN when it exists (bit for
+! bit the continuation of the run that wrote it) instead of the float32
+! frame.
 program gturbo_main
   use iso_c_binding
   use gturbo_hip
@@ -19,11 +23,18 @@ program gturbo_main
   integer(c_int64_t) :: nbytes
   character(len=64) :: nome, arg
   real :: s
+  logical :: exact, spectral
+  logical, external :: havez
 
   call system_clock(start_time, trate)
   if (command_argument_count() >= 1) then
     call get_command_argument(1, arg)
     read(arg, *) nx
+  end if
+  exact = .false.
+  if (command_argument_count() >= 2) then
+    call get_command_argument(2, arg)
+    exact = trim(arg) == 'exact'
   end if
   call show_mem(fre, tota, 0)
 
@@ -39,6 +50,12 @@ program gturbo_main
   ns = (nx + 2) * nx
   allocate(z(ns), znl(ns), znew(ns), u(ns), v(ns))
   call inieuler(z, ifr)
+  spectral = exact .and. ifr > 0
+  if (spectral) spectral = havez(ifr)
+  if (spectral) then
+    call readz(z, ifr)
+    write(6, *) ' Resuming from the FP64 state of frame ', ifr
+  end if
   call gtcheck(gt_plan_create(gtctx, GT_PLAN_Z2D, plan_inv), 'plan Z2D')
   call gtcheck(gt_plan_create(gtctx, GT_PLAN_D2Z, plan_dir), 'plan D2Z')
 
@@ -59,9 +76,11 @@ program gturbo_main
   call gtcheck(gt_enter_data(gtctx, z, nbytes, 1), 'enter z')
   call show_mem(fre, tota, 2)
 
-  ! src/main.f90:89-92
-  call fft_dir(z, plan_dir)
-  if (truncate == 1) call trunc(z)
+  ! src/main.f90:89-92 (an FP64 spectral state is already transformed)
+  if (.not. spectral) then
+    call fft_dir(z, plan_dir)
+    if (truncate == 1) call trunc(z)
+  end if
 
   jfr = 1
   do ipas = 1, npas                                      ! src/main.f90:97-137
@@ -92,6 +111,8 @@ program gturbo_main
 
   ! src/main.f90:140-154
   ifr = ifr + 1
+  call gtcheck(gt_update_host(gtctx, z), 'update host z')
+  call writez(z, ifr)
   call fft_inv(z, plan_inv)
   write(nome, "('./files/seed.',i3.3)") ifr
   open(unit=2, file=nome, status='replace')

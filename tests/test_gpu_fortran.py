@@ -140,3 +140,29 @@ def test_fortran_initial_condition_inversion(tmp_path):
     assert all(b < a for a, b in zip(losses, losses[1:]))
     assert losses[-1] < 0.5 * losses[0]
     assert os.path.exists(tmp_path / "fields" / "winv.001")
+
+
+def test_fortran_fp64_checkpoint_resume_is_exact(tmp_path):
+    """Two chained 10-step segments resumed from the FP64 state fields/z.002
+    ("exact") end bit-identical to one 20-step segment: same FP64 state,
+    same float32 frame, same seed."""
+    n = 64
+    base = Params(n=n, nu=1e-12, alpnu=4.0, mu=0.1, alpmu=0.0, alpv=1.0, dt=2e-3, rko=4,
+                  npas=20, iout=10**6, imix=10**6, famp=1.0, k1f=3.0, k2f=5.0)
+    one = tmp_path / "one"
+    two = tmp_path / "two"
+    _setup(one, base)
+    _run(one, n)
+    _setup(two, base.replace(npas=10))
+    _run(two, n)
+    r = subprocess.run([EXE, str(n), "exact"], cwd=str(two), capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Resuming from the FP64 state" in r.stdout
+    za = np.fromfile(one / "fields" / "z.002", dtype=np.uint8)
+    zb = np.fromfile(two / "fields" / "z.003", dtype=np.uint8)
+    assert za.size == zb.size and np.array_equal(za, zb)
+    wa = io.read_field(str(one / "fields" / "w.002"), n)
+    wb = io.read_field(str(two / "fields" / "w.003"), n)
+    assert np.array_equal(wa, wb)
+    assert io.read_seed(str(one / "files" / "seed.002")) == io.read_seed(str(two / "files" / "seed.003"))
