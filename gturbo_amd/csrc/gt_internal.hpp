@@ -108,6 +108,20 @@ __host__ __device__ constexpr int rp_of(int logn) {
        : logn == 9 ? 8 : logn == 10 ? 4 : logn == 11 ? 2 : 1;
 }
 
+// Layout of the fields between the passes: rows in groups of kYR; group g
+// of a field holds, tile after tile, a block of kYR rows x BT columns.  The
+// y pass writes whole blocks (full cache lines); the x pass reads a row
+// pair's pieces of consecutive tiles from one DRAM page.  Measured at
+// 4096^2 (BT = 2): kYR = 4 beats whole-column tiles, 8 and 2 (2 leaves the
+// y pass writing half lines).
+constexpr int kYR = 4;
+__host__ __device__ inline unsigned yrow_off(int y, int ntiles, int bt) {
+  return (unsigned)(y / kYR) * (unsigned)(ntiles * kYR * bt) + (unsigned)(y % kYR * bt);
+}
+__host__ __device__ inline unsigned ycol_off(int c, int bt) {
+  return (unsigned)(c / bt) * (unsigned)(kYR * bt) + (unsigned)(c % bt);
+}
+
 // Compact row r of FFT index p (ky in FFT order), or -1 outside the rect.
 __host__ __device__ inline int row_of(int p, const Rect& rc, int n) {
   if (p <= rc.p) return p;

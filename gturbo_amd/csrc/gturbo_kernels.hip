@@ -13,11 +13,11 @@
 //    the products q u, q v (src/produ.f90), and one real-to-complex transform
 //    of q u + i q v.
 //
-// Between the passes the three (resp. two) fields live in HBM in a tiled
-// layout [field][tile][y][kx % BT]: the y pass reads and writes whole tiles
-// contiguously; the x pass reads and writes BT*16-byte runs, and the
-// workgroup order keeps neighbouring rows on one XCD so that their runs
-// share L2 lines.
+// Between the passes the three (resp. two) fields live in HBM in blocks of
+// kYR rows x BT columns, [field][y / kYR][tile][y % kYR][kx % BT]: the y
+// pass writes whole blocks (a 128-byte line at N = 4096), the x pass reads a
+// row's pieces of consecutive tiles from one DRAM page, and its workgroup
+// order keeps neighbouring rows on one XCD so that they share L2 lines.
 #include "fft_core.hpp"
 #include "gt_internal.hpp"
 
@@ -139,15 +139,16 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
 
   if constexpr (FWD) {
     double2 A[kE];
-    const double2* srcA = a.y2 + (size_t)tile * N * BT;
+    const unsigned colo = ycol_off(c, BT);
+    const double2* srcA = a.y2;
 #pragma unroll
-    for (int r = 0; r < kE; ++r) z[r] = srcA[(unsigned)((j + TPL * r) * BT + b)];
+    for (int r = 0; r < kE; ++r) z[r] = srcA[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
     fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
 #pragma unroll
     for (int r = 0; r < kE; ++r) A[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
-    const double2* srcB = a.y2 + (size_t)(a.ntiles + tile) * N * BT;
+    const double2* srcB = a.y2 + (size_t)a.ntiles * N * BT;
 #pragma unroll
-    for (int r = 0; r < kE; ++r) z[r] = srcB[(unsigned)((j + TPL * r) * BT + b)];
+    for (int r = 0; r < kE; ++r) z[r] = srcB[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
     fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
     int je = j;
     // The epilogue's indices (and so its loads) depend on the transform's
@@ -275,9 +276,10 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
         z[r] = v;
       }
       fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
-      double2* dst = a.y1 + (size_t)(f * a.ntiles + tile) * N * BT;
+      double2* dst = a.y1 + (size_t)f * a.ntiles * N * BT;
+      const unsigned colo = ycol_off(c, BT);
 #pragma unroll
-      for (int r = 0; r < kE; ++r) dst[(unsigned)((j + TPL * r) * BT + b)] = z[r];
+      for (int r = 0; r < kE; ++r) dst[yrow_off(j + TPL * r, a.ntiles, BT) + colo] = z[r];
     }
   }
 }
@@ -303,8 +305,7 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
 // Offset of column k of a y-row in the tiled layout [tile][y][k % BT].
 template <int LOGN, int BT>
 __device__ __forceinline__ unsigned koff(int k) {
-  constexpr int N = 1 << LOGN;
-  return (unsigned)(k / BT) * (unsigned)(N * BT) + (unsigned)(k % BT);
+  return ycol_off(k, BT);
 }
 
 template <int LOGN, int TR, int BT>
@@ -369,7 +370,8 @@ __device__ __forceinline__ void r_body(const PassArgs& a, int g, int tid,
   double2* buf = lds + pl * LINE;
   const int kxm = a.rc.kx;
   const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field
-  const unsigned offQ = (unsigned)yrow * BT, offU = offQ + fstride, offV = offU + fstride;
+  const unsigned offQ = yrow_off(yrow, a.ntiles, BT), offU = offQ + fstride,
+                 offV = offU + fstride;
 
   double2 z[kE];
   double qv[kE];
@@ -401,7 +403,7 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   const int y0 = 2 * (g * NL + pl);
   const int kxm = a.rc.kx;
   const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field
-  const unsigned r0 = (unsigned)y0 * BT, r1 = r0 + BT;    // rows y0, y0+1
+  const unsigned r0 = yrow_off(y0, a.ntiles, BT), r1 = r0 + BT;  // rows y0, y0+1
   const unsigned fq = 0, fu = fstride, fv = 2 * fstride;
 
   double2 z[kE], qq[kE];
@@ -601,15 +603,16 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
 
   if constexpr (FWD) {
     double2 C[kE];
-    const double2* y2 = a.y2 + (size_t)tile * N * BT;
+    const double2* y2 = a.y2;
     const size_t fs = (size_t)a.ntiles * N * BT;
+    const unsigned colo = ycol_off(c, BT);
 #pragma unroll
-    for (int r = 0; r < kE; ++r) z[r] = y2[fs + (unsigned)((j + TPL * r) * BT + b)];
+    for (int r = 0; r < kE; ++r) z[r] = y2[fs + yrow_off(j + TPL * r, a.ntiles, BT) + colo];
     fft_reg<LOGN, -1>(z, buf, j, tw);  // F(q lambda_x)
 #pragma unroll
     for (int r = 0; r < kE; ++r) C[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
 #pragma unroll
-    for (int r = 0; r < kE; ++r) z[r] = y2[2 * fs + (unsigned)((j + TPL * r) * BT + b)];
+    for (int r = 0; r < kE; ++r) z[r] = y2[2 * fs + yrow_off(j + TPL * r, a.ntiles, BT) + colo];
     fft_reg<LOGN, -1>(z, buf, j, tw);  // F(q lambda_y)
     int jc = j;
     asm volatile("" : "+v"(jc) : "v"(z[0].x), "v"(z[kE - 1].y));
@@ -627,7 +630,7 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
       C[r] = v;
     }
 #pragma unroll
-    for (int r = 0; r < kE; ++r) z[r] = y2[(unsigned)((j + TPL * r) * BT + b)];
+    for (int r = 0; r < kE; ++r) z[r] = y2[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
     fft_reg<LOGN, -1>(z, buf, j, tw);  // F(u lambda_x + v lambda_y)
     int je = j;
     asm volatile("" : "+v"(je) : "v"(z[0].x), "v"(z[kE - 1].y));
@@ -727,9 +730,10 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
         z[r] = v;
       }
       fft_reg<LOGN, +1>(z, buf, j, tw);
-      double2* dst = a.y1 + (size_t)(f * a.ntiles + tile) * N * BT;
+      double2* dst = a.y1 + (size_t)f * a.ntiles * N * BT;
+      const unsigned colo = ycol_off(c, BT);
 #pragma unroll
-      for (int r = 0; r < kE; ++r) dst[(unsigned)((j + TPL * r) * BT + b)] = z[r];
+      for (int r = 0; r < kE; ++r) dst[yrow_off(j + TPL * r, a.ntiles, BT) + colo] = z[r];
     }
   }
 }
@@ -771,7 +775,7 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
   double2* buf = lds + pl * LINE;
   const int kxm = a.rc.kx;
   const unsigned fs = (unsigned)a.ntiles * N * BT;
-  const unsigned off = (unsigned)yrow * BT;
+  const unsigned off = yrow_off(yrow, a.ntiles, BT);
 
   double2 z[kE], l[kE];
   double ap[kE];
