@@ -110,6 +110,11 @@ __device__ __forceinline__ void dft(double2* a) {
 }
 
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 4); }
+// lpad(e + c) - lpad(e) for a power-of-two-aligned run: valid whenever c is a
+// multiple of a power of two P and e mod P + c lies in the same 16-element
+// block as c (the transposes below only form such sums), so one base address
+// per thread plus compile-time offsets replaces a padded index per access.
+__host__ __device__ constexpr int lpad_off(int c) { return c + c / 16; }
 
 template <int LOGN>
 struct Line {
@@ -190,15 +195,18 @@ __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
 #pragma unroll
       for (int r = 0; r < R; ++r) a[b + NB * r] = t[r];
     } else {
-      const int idx = (jj / NS) * NS * R + k;
+      // idx = (jj/NS) NS R + k with k < NS: idx + r NS never crosses into
+      // another 16-element block than r NS does, so lpad splits.
+      double2* p = buf + lpad((jj / NS) * NS * R + k);
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[lpad(idx + r * NS)] = t[r];
+      for (int r = 0; r < R; ++r) p[lpad_off(r * NS)] = t[r];
     }
   }
   if constexpr (!LAST) {
     __syncthreads();
+    const double2* p = buf + lpad(j);  // j < TPL, TPL a power of two
 #pragma unroll
-    for (int r = 0; r < kE; ++r) a[r] = buf[lpad(j + TPL * r)];
+    for (int r = 0; r < kE; ++r) a[r] = p[lpad_off(TPL * r)];
     __syncthreads();
   }
 }

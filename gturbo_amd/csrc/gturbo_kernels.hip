@@ -328,6 +328,21 @@ __device__ __forceinline__ void build_two(double2* z, int j,
     z[r] = v;
   }
 }
+// Element (N - k) mod N, k = j + TPL r (r a constant), of a line in LDS;
+// pm = buf + lpad(TPL - j).  For TPL >= 16 the index TPL (7 - r) + TPL - j
+// splits into that base and a constant offset; k = 0 maps to element 0.
+template <int LOGN>
+__device__ __forceinline__ double2 mirror_at(const double2* buf, const double2* pm, int j,
+                                             int r) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
+  if constexpr (TPL % 16 != 0) {
+    return buf[lpad((N - j - TPL * r) & (N - 1))];
+  } else {
+    if (r == 0) return (j == 0 ? buf : pm + lpad_off(7 * TPL))[0];
+    return pm[lpad_off(TPL * (7 - r))];
+  }
+}
+
 // Split W = FFT(a + i b) (held at k = j + TPL*r) into the half-spectra of
 // the real sequences a and b and store kx <= kx_max.  W[N-k] comes through
 // LDS.
@@ -336,16 +351,18 @@ __device__ __forceinline__ void split_store(const double2* w, double2* buf, int 
                                             double2* __restrict__ y,
                                             unsigned off_a, unsigned off_b, int kxm) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
+  double2* pj = buf + lpad(j);
 #pragma unroll
   for (int r = 0; r < kE; ++r)
-    if (live_r<TR>(r)) buf[lpad(j + TPL * r)] = w[r];
+    if (live_r<TR>(r)) pj[lpad_off(TPL * r)] = w[r];
   __syncthreads();
+  const double2* pm = buf + lpad(TPL - j);
 #pragma unroll
   for (int r = 0; r <= kE / 2; ++r) {  // k <= N/2 (r = kE/2 only for k = N/2)
     const int k = j + TPL * r;
     if (live_r<TR>(r) && k <= kxm && k <= N / 2) {
       const double2 W = w[r];
-      const double2 Wc = buf[lpad((N - k) & (N - 1))];
+      const double2 Wc = mirror_at<LOGN>(buf, pm, j, r);
       y[off_a + koff<LOGN, BT>(k)] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
       y[off_b + koff<LOGN, BT>(k)] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
     }
@@ -471,9 +488,11 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / kE)
     w[r] = make_double2(r0[x], r1[x]);
   }
   fft_reg<LOGN, -1>(w, buf, j, tw);
+  double2* pj = buf + lpad(j);
 #pragma unroll
-  for (int r = 0; r < kE; ++r) buf[lpad(j + TPL * r)] = w[r];
+  for (int r = 0; r < kE; ++r) pj[lpad_off(TPL * r)] = w[r];
   __syncthreads();  // also orders every read of the real rows before the writes
+  const double2* pm = buf + lpad(TPL - j);
   double2* c0 = reinterpret_cast<double2*>(z + ld * y0);
   double2* c1 = c0 + (N / 2 + 1);
 #pragma unroll
@@ -481,7 +500,7 @@ __global__ void __launch_bounds__(rp_of(LOGN) * (1 << LOGN) / kE)
     const int k = j + TPL * r;
     if (k <= N / 2) {
       const double2 W = w[r];
-      const double2 Wc = buf[lpad((N - k) & (N - 1))];
+      const double2 Wc = mirror_at<LOGN>(buf, pm, j, r);
       c0[k] = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
       c1[k] = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
     }
