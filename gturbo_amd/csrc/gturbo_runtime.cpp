@@ -60,8 +60,12 @@ hipEvent_t get_event(gt_ctx* c) {
     c->ev_pool.pop_back();
     return e;
   }
+  // Timing only: without the system-scope fence a record does not write
+  // back and invalidate the caches between the kernels it separates (which
+  // would slow the kernels being timed).  gt_timing_read synchronizes the
+  // stream before it reads the events.
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
