@@ -298,35 +298,53 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
 // x pass
 // ---------------------------------------------------------------------------
 
-// Z = F + i G over the full x-spectrum at the thread's positions
-// p = j + TPL*r, from the half-spectra F, G of two real sequences (kx <=
-// kx_max; the imaginary parts at kx = 0 and N/2 are not used, as in a C2R
-// transform).  The inverse transform of Z is f + i g.
-// Offset of column k of a y-row in the tiled layout [tile][y][k % BT].
+// Offset of column k of a y-row in the blocked layout.
 template <int LOGN, int BT>
 __device__ __forceinline__ unsigned koff(int k) {
   return ycol_off(k, BT);
 }
 
+// Z = F + i G over the full x-spectrum at the thread's positions
+// p = j + TPL*r, from the half-spectra F, G of two real sequences (kx <=
+// kx_max; the imaginary parts at kx = 0 and N/2 are not used, as in a C2R
+// transform).  The inverse transform of Z is f + i g.  Built in two
+// halves: the loads (zeros outside the kept half-spectrum), all issued
+// before any is used so that they are in flight together, and the packing.
 template <int LOGN, int TR, int BT>
-__device__ __forceinline__ void build_two(double2* z, int j,
-                                          const double2* __restrict__ y,
-                                          unsigned off_f, unsigned off_g, int kxm) {
+__device__ __forceinline__ void load_two(double2* x0, double2* x1, int j,
+                                         const double2* __restrict__ y,
+                                         unsigned off_f, unsigned off_g, int kxm) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
 #pragma unroll
   for (int r = 0; r < kE; ++r) {
     const int p = j + TPL * r;
-    double2 v = make_double2(0.0, 0.0);
     const int k = p <= N / 2 ? p : N - p;
+    x0[r] = x1[r] = make_double2(0.0, 0.0);
     if (live_r<TR>(r) && k <= kxm) {
-      const double2 x0 = y[off_f + koff<LOGN, BT>(k)];
-      const double2 x1 = y[off_g + koff<LOGN, BT>(k)];
-      if (p == 0 || p == N / 2) v = make_double2(x0.x, x1.x);
-      else if (p < N / 2) v = make_double2(x0.x - x1.y, x0.y + x1.x);
-      else v = make_double2(x0.x + x1.y, x1.x - x0.y);
+      x0[r] = y[off_f + koff<LOGN, BT>(k)];
+      x1[r] = y[off_g + koff<LOGN, BT>(k)];
     }
-    z[r] = v;
   }
+}
+template <int LOGN>
+__device__ __forceinline__ void pack_two(double2* z, int j, const double2* x0,
+                                         const double2* x1) {
+  constexpr int N = 1 << LOGN, TPL = N / kE;
+#pragma unroll
+  for (int r = 0; r < kE; ++r) {
+    const int p = j + TPL * r;
+    if (p == 0 || p == N / 2) z[r] = make_double2(x0[r].x, x1[r].x);
+    else if (p < N / 2) z[r] = make_double2(x0[r].x - x1[r].y, x0[r].y + x1[r].x);
+    else z[r] = make_double2(x0[r].x + x1[r].y, x1[r].x - x0[r].y);
+  }
+}
+template <int LOGN, int TR, int BT>
+__device__ __forceinline__ void build_two(double2* z, int j,
+                                          const double2* __restrict__ y,
+                                          unsigned off_f, unsigned off_g, int kxm) {
+  double2 x0[kE], x1[kE];
+  load_two<LOGN, TR, BT>(x0, x1, j, y, off_f, off_g, kxm);
+  pack_two<LOGN>(z, j, x0, x1);
 }
 // Element (N - k) mod N, k = j + TPL r (r a constant), of a line in LDS;
 // pm = buf + lpad(TPL - j).  For TPL >= 16 the index TPL (7 - r) + TPL - j
@@ -455,7 +473,7 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
 
 // One row pair per workgroup: two workgroups share a CU.
 template <int LOGN, int TR>
-__global__ void __launch_bounds__((1 << LOGN) / kE)
+__global__ void __launch_bounds__((1 << LOGN) / kE, 4)
     r_pass_pair(PassArgs a) {
   // Twiddles from memory: these workgroups are short-lived (one row pair),
   // and staging the table in LDS measured slower here.
