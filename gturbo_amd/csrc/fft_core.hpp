@@ -153,14 +153,24 @@ __device__ __forceinline__ const double2* stage_tw(double2* twl, const double2* 
   }
 }
 
+// W^k of a radix-8 pass of stride NS for the butterfly of thread j.
+template <int LOGN, int NS>
+__device__ __forceinline__ double2 tw_of(const double2* tw, int j) {
+  return tw[(j % NS) * ((1 << LOGN) / (NS * 8))];
+}
+
 // One Stockham pass: radix R, stride Ns.  Butterfly jj (< N/R) reads
-// x[jj + (N/R) r'] and writes y[(jj/Ns) Ns R + jj%Ns + Ns r'].
-template <int LOGN, int S, int R, int NS, bool LAST>
+// x[jj + (N/R) r'] and writes y[(jj/Ns) Ns R + jj%Ns + Ns r'].  With PRE the
+// pass takes its W^k from wc (loaded by the previous pass); with NSN > 0 it
+// loads the next radix-8 pass's W^k into wc before its transpose, so that
+// the load is in flight across the barriers.
+template <int LOGN, int S, int R, int NS, bool LAST, bool PRE = false, int NSN = 0>
 __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
-                                         const double2* tw) {
+                                         const double2* tw, double2& wc) {
   constexpr int N = 1 << LOGN;
   constexpr int TPL = N / kE;
   constexpr int NB = kE / R;  // butterflies per thread
+  static_assert(!PRE || (R == kE && NS > 1), "a carried twiddle needs one butterfly");
   // Opaque copy of the thread index: the addresses below are then computed
   // inside each pass instead of being shared (and kept live) across every
   // transform of the kernel.
@@ -176,7 +186,8 @@ __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
       // W^(r k), r < R, from the table entries for r = 1, 2, 4
       constexpr int ST = N / (NS * R);
       double2 w[R];
-      w[1] = tw[k * ST];
+      if constexpr (PRE) w[1] = wc;
+      else w[1] = tw[k * ST];
       if constexpr (R >= 4) {
         w[2] = cmul(w[1], w[1]);
         w[3] = cmul(w[1], w[2]);
@@ -191,6 +202,9 @@ __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
       for (int r = 1; r < R; ++r) t[r] = S < 0 ? cmul(t[r], w[r]) : cmulc(t[r], w[r]);
     }
     dft<R, S>(t);
+    if constexpr (NSN > 0) {
+      if (b == NB - 1) wc = tw_of<LOGN, NSN>(tw, j);
+    }
     if constexpr (LAST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) a[b + NB * r] = t[r];
@@ -214,30 +228,38 @@ __device__ __forceinline__ void reg_pass(double2* a, double2* buf, int j,
 // Transform of one line held in registers; S = -1 forward, +1 inverse
 // (unnormalised).  Every thread of the workgroup must call it (the
 // transposes hold barriers); j is the thread's index within its line.
+// TWPF: each radix-8 pass loads the next one's twiddle before its transpose
+// (for a table in memory, whose load latency the barriers then hide; a
+// table in LDS gains nothing from it and pays the extra registers).
 // SCHED picks the radix order when log2 N is not a multiple of 3: 0 runs
 // the radix-8 passes first and the radix-2/4 pass last, 1 the other way
 // round (same result up to rounding; the faster one depends on N).
-template <int LOGN, int S, int SCHED = 0>
+template <int LOGN, int S, int SCHED = 0, bool TWPF = false>
 __device__ __forceinline__ void fft_reg(double2* a, double2* buf, int j,
                                         const double2* tw) {
   using L = Line<LOGN>;
   constexpr int NP = L::NPASS8 + (L::RLAST > 1 ? 1 : 0);
+  constexpr int P8 = L::NPASS8;
+  constexpr bool P = TWPF;
+  // next radix-8 stride when this pass hands a twiddle on, else 0
+  constexpr auto nx = [](bool has_next, int ns) { return TWPF && has_next ? ns : 0; };
+  double2 wc;  // W^k of the next radix-8 pass, carried across a transpose
   if constexpr (SCHED == 0 || L::RLAST == 1) {
-    if constexpr (L::NPASS8 >= 1) reg_pass<LOGN, S, 8, 1, NP == 1>(a, buf, j, tw);
-    if constexpr (L::NPASS8 >= 2) reg_pass<LOGN, S, 8, 8, NP == 2>(a, buf, j, tw);
-    if constexpr (L::NPASS8 >= 3) reg_pass<LOGN, S, 8, 64, NP == 3>(a, buf, j, tw);
-    if constexpr (L::NPASS8 >= 4) reg_pass<LOGN, S, 8, 512, NP == 4>(a, buf, j, tw);
+    if constexpr (P8 >= 1) reg_pass<LOGN, S, 8, 1, NP == 1, false, nx(P8 >= 2, 8)>(a, buf, j, tw, wc);
+    if constexpr (P8 >= 2) reg_pass<LOGN, S, 8, 8, NP == 2, P, nx(P8 >= 3, 64)>(a, buf, j, tw, wc);
+    if constexpr (P8 >= 3) reg_pass<LOGN, S, 8, 64, NP == 3, P, nx(P8 >= 4, 512)>(a, buf, j, tw, wc);
+    if constexpr (P8 >= 4) reg_pass<LOGN, S, 8, 512, NP == 4, P, 0>(a, buf, j, tw, wc);
     if constexpr (L::RLAST > 1) {
-      constexpr int NS = 1 << (3 * L::NPASS8);
-      reg_pass<LOGN, S, L::RLAST, NS, true>(a, buf, j, tw);
+      constexpr int NS = 1 << (3 * P8);
+      reg_pass<LOGN, S, L::RLAST, NS, true>(a, buf, j, tw, wc);
     }
   } else {
     constexpr int R0 = L::RLAST;
-    reg_pass<LOGN, S, R0, 1, NP == 1>(a, buf, j, tw);
-    if constexpr (L::NPASS8 >= 1) reg_pass<LOGN, S, 8, R0, NP == 2>(a, buf, j, tw);
-    if constexpr (L::NPASS8 >= 2) reg_pass<LOGN, S, 8, R0 * 8, NP == 3>(a, buf, j, tw);
-    if constexpr (L::NPASS8 >= 3) reg_pass<LOGN, S, 8, R0 * 64, NP == 4>(a, buf, j, tw);
-    if constexpr (L::NPASS8 >= 4) reg_pass<LOGN, S, 8, R0 * 512, NP == 5>(a, buf, j, tw);
+    reg_pass<LOGN, S, R0, 1, NP == 1>(a, buf, j, tw, wc);
+    if constexpr (P8 >= 1) reg_pass<LOGN, S, 8, R0, NP == 2, false, nx(P8 >= 2, R0 * 8)>(a, buf, j, tw, wc);
+    if constexpr (P8 >= 2) reg_pass<LOGN, S, 8, R0 * 8, NP == 3, P, nx(P8 >= 3, R0 * 64)>(a, buf, j, tw, wc);
+    if constexpr (P8 >= 3) reg_pass<LOGN, S, 8, R0 * 64, NP == 4, P, nx(P8 >= 4, R0 * 512)>(a, buf, j, tw, wc);
+    if constexpr (P8 >= 4) reg_pass<LOGN, S, 8, R0 * 512, NP == 5, P, 0>(a, buf, j, tw, wc);
   }
 }
 

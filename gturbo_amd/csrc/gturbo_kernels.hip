@@ -427,7 +427,7 @@ __device__ __forceinline__ void r_body(const PassArgs& a, int g, int tid,
 // of both rows; V0 + iV1 and U0 + iU1 likewise, and the products of the two
 // rows are again packed two-for-one for the forward transforms: five
 // transforms per row pair, only q carried across them.
-template <int LOGN, int TR, int BT, int NL, int SCHED>
+template <int LOGN, int TR, int BT, int NL, int SCHED, bool TWPF = false>
 __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
                                           double2* lds, const double2* tw) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
@@ -443,20 +443,20 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
 
   double2 z[kE], qq[kE];
   build_two<LOGN, TR, BT>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
+  fft_reg<LOGN, +1, SCHED, TWPF>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) qq[r] = z[r];                   // q0 + i q1
   build_two<LOGN, TR, BT>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
+  fft_reg<LOGN, +1, SCHED, TWPF>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);                   // (q v)0 + i (q v)1
+  fft_reg<LOGN, -1, SCHED, TWPF>(z, buf, j, tw);                   // (q v)0 + i (q v)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, fstride + r0, fstride + r1, kxm);
   build_two<LOGN, TR, BT>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
-  fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
+  fft_reg<LOGN, +1, SCHED, TWPF>(z, buf, j, tw);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);                   // (q u)0 + i (q u)1
+  fft_reg<LOGN, -1, SCHED, TWPF>(z, buf, j, tw);                   // (q u)0 + i (q u)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, r0, r1, kxm);
 }
 
@@ -471,15 +471,16 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
                                        tw);
 }
 
-// One row pair per workgroup: two workgroups share a CU.
+// One row pair per workgroup: two workgroups share a CU.  Twiddles come
+// from memory, each transform pass loading the next pass's entry ahead.
 template <int LOGN, int TR>
 __global__ void __launch_bounds__((1 << LOGN) / kE, 4)
     r_pass_pair(PassArgs a) {
   // Twiddles from memory: these workgroups are short-lived (one row pair),
   // and staging the table in LDS measured slower here.
   __shared__ double2 lds[Line<LOGN>::PADDED];
-  pair_body<LOGN, TR, bt_of(LOGN), 1, 0>(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x,
-                                         lds, a.tw);
+  pair_body<LOGN, TR, bt_of(LOGN), 1, 0, true>(a, xcd_remap(blockIdx.x, gridDim.x),
+                                               threadIdx.x, lds, a.tw);
 }
 
 // ---------------------------------------------------------------------------
