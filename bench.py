@@ -253,8 +253,8 @@ def main():
                          "field (a plain copy of known size for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the 2D pseudospectral "
-                          "step is not sharded across GPUs (see DESIGN.md); "
+        print(json.dumps({"error": "gturbo_amd runs on one GPU: the 2D pseudospectral "
+                          "step is not sharded across GPUs; "
                           f"--gpus {args.gpus} is not supported"}))
         sys.exit(2)
 

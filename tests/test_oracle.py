@@ -176,7 +176,7 @@ def test_trunc_mask_exact_rational(O, n, count):
     exact[0, 0] = False
     assert np.array_equal(kept, exact)
     if count is not None:
-        assert int(kept.sum()) + 1 == count  # the survey's count includes the mean
+        assert int(kept.sum()) + 1 == count  # the tabulated count includes the mean
 
 
 @pytest.mark.parametrize("n,ring,nf", [(256, (15, 17), 96), (512, (31, 33), 204),
