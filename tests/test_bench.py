@@ -1,0 +1,66 @@
+"""bench.py's output contract: one JSON line with the headline metric, the
+roofline entry and the CPU baseline; one GPU only."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BENCH = os.path.join(ROOT, "bench.py")
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, BENCH, "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in out["error"]
+
+
+def test_committed_counters_match_the_byte_model():
+    """The HBM bytes per launch measured with FETCH_SIZE/WRITE_SIZE (the
+    committed summary) agree with the algorithmic bytes of each kernel
+    class within 2%: the passes move no more than they must."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from gturbo_amd import workloads
+    w = workloads()["euler4096_rk4"]
+    n = w.params.n
+    m = n // 3  # the 2/3 rectangle: kx, ky in [-N/3, N/3]
+    ab = bench.algorithmic_bytes(n, (m, m, m), 4)
+    for k, v in ab.items():
+        t = bench.read_traffic(k)
+        assert t is not None, k
+        assert abs(t / v - 1.0) < 0.02, (k, t, v)
+    # one RK4 step: y_start + 3 y_stage + y_final + 4 x passes
+    per_step = ab["y_start"] + 3 * ab["y_stage"] + ab["y_final"] + 4 * ab["x_pass"]
+    assert 250 < per_step / n ** 2 < 265
+
+
+@pytest.mark.gpu
+def test_bench_line_has_the_contract_keys():
+    r = subprocess.run([sys.executable, BENCH, "--n", "256", "--steps", "3", "--warmup", "1",
+                        "--rk2-steps", "2", "--ens-steps", "2"],
+                       capture_output=True, text=True, timeout=600, cwd=ROOT)
+    print(r.stderr[-3000:])
+    assert r.returncode == 0
+    lines = [ln for ln in r.stdout.strip().splitlines() if ln.strip()]
+    assert len(lines) == 1
+    d = json.loads(lines[0])
+    for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+              "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",
+              "roofline", "cpu_baseline"):
+        assert k in d, k
+    assert d["n_gpus"] == 1 and d["steps"] == 3 and d["warmup"] == 1
+    assert d["value"] > 0 and d["dtype"] == "f64" and d["higher_is_better"] is True
+    assert "workload" in d["config"]
+    roof = d["roofline"]
+    assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
+    assert abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-12
+    assert roof["traffic"] is None  # the counter summary is of the 4096^2 workload only
+    cb = d["cpu_baseline"]
+    assert cb["value"] > 0 and cb["kind"] == "port" and cb["cores"] >= 1 and cb["sample"]
+    for leg in ("rk2", "ensemble", "adjoint"):
+        assert d[leg]["finite"] is True, leg
