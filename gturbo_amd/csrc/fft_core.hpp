@@ -263,4 +263,141 @@ __device__ __forceinline__ void fft_reg(double2* a, double2* buf, int j,
   }
 }
 
+// ---------------------------------------------------------------------------
+// In-place radix-8 transforms of a 4096-point line over 512 threads (eight
+// whole waves, thread j = lane + 64 wave).  The first pass combines the
+// elements j + 512 r that thread j holds; its outputs go to LDS, and each
+// later pass reads, transforms and writes back one run of the line that
+// belongs to a single wave (stride 64: wave j/64's 512 elements; stride 8
+// and 1: runs inside them).  Only the first exchange crosses waves, so a
+// transform needs one workgroup barrier plus one before its first write
+// (when LDS may still be read by another wave), instead of two per pass.
+//   fft_dif: natural order in, out of order: thread j, register r holds
+//            X[D(j) + 512 r], D(j) = 64 (j mod 8) + 8 (j/8 mod 8) + j/64
+//            (the octal digits of j reversed).
+//   fft_dit: the transposed algorithm (same twiddles, applied before each
+//            radix-8 DFT, passes in reverse): input in that order, natural
+//            order out.
+// LDS element e sits at e + e/8: every pass then addresses one base per
+// thread plus constants, and a wave's accesses at stride 8 or 9 spread over
+// the banks.
+// ---------------------------------------------------------------------------
+constexpr int kIpLine = 4096 + 4096 / 8;  // double2 per line in LDS
+
+__device__ __forceinline__ int ip_pad(int e) { return e + (e >> 3); }
+
+// Orders a wave's LDS writes before its later reads (LDS executes one wave's
+// operations in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// t[r] *= W^(k r) (conjugated for S > 0), from w1 = W^k.
+template <int S>
+__device__ __forceinline__ void apply_tw8(double2* t, double2 w1) {
+  double2 w[8];
+  w[1] = w1;
+  w[2] = cmul(w[1], w[1]);
+  w[3] = cmul(w[1], w[2]);
+  w[4] = cmul(w[2], w[2]);
+  w[5] = cmul(w[1], w[4]);
+  w[6] = cmul(w[2], w[4]);
+  w[7] = cmul(w[3], w[4]);
+#pragma unroll
+  for (int r = 1; r < 8; ++r) t[r] = S < 0 ? cmul(t[r], w[r]) : cmulc(t[r], w[r]);
+}
+
+// BAR: start with a workgroup barrier (another wave may still read LDS).
+template <int S, bool BAR>
+__device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
+                                        const double2* __restrict__ tw) {
+  asm volatile("" : "+v"(j));
+  double2 wn = tw[j];  // pass 1: W^(j r)
+  dft<8, S>(a);
+  apply_tw8<S>(a, wn);
+  wn = tw[8 * (j & 63)];  // pass 2, in flight across the barrier
+  if constexpr (BAR) __syncthreads();
+  {
+    double2* p = buf + ip_pad(j);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[576 * r] = a[r];  // ip_pad(j + 512 r)
+  }
+  __syncthreads();
+  {  // pass 2, stride 64: wave j/64's elements
+    double2* p = buf + ip_pad((j >> 6) * 512 + (j & 63));
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[r] = p[72 * r];
+    dft<8, S>(a);
+    apply_tw8<S>(a, wn);
+    wn = tw[64 * (j & 7)];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[72 * r] = a[r];
+  }
+  wave_sync();
+  {  // pass 3, stride 8
+    double2* p = buf + ip_pad((j >> 3) * 64 + (j & 7));
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[r] = p[9 * r];
+    dft<8, S>(a);
+    apply_tw8<S>(a, wn);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[9 * r] = a[r];
+  }
+  wave_sync();
+  {  // pass 4, stride 1
+    const double2* p = buf + 9 * j;  // ip_pad(8 j)
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[r] = p[r];
+    dft<8, S>(a);
+  }
+}
+
+// Input as fft_dif leaves it; the thread's own LDS run is free to write
+// (its last reader was this wave).  Ends after reads of every wave's
+// elements: a barrier must precede the next LDS write.
+template <int S>
+__device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
+                                        const double2* __restrict__ tw) {
+  asm volatile("" : "+v"(j));
+  double2 wn = tw[64 * (j & 7)];
+  dft<8, S>(a);  // pass 4, stride 1
+  {
+    double2* p = buf + 9 * j;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[r] = a[r];
+  }
+  wave_sync();
+  {  // pass 3, stride 8
+    double2* p = buf + ip_pad((j >> 3) * 64 + (j & 7));
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[r] = p[9 * r];
+    apply_tw8<S>(a, wn);
+    wn = tw[8 * (j & 63)];
+    dft<8, S>(a);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[9 * r] = a[r];
+  }
+  wave_sync();
+  {  // pass 2, stride 64
+    double2* p = buf + ip_pad((j >> 6) * 512 + (j & 63));
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[r] = p[72 * r];
+    apply_tw8<S>(a, wn);
+    wn = tw[j];
+    dft<8, S>(a);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[72 * r] = a[r];
+  }
+  __syncthreads();
+  {  // pass 1, stride 512: natural order out
+    const double2* p = buf + ip_pad(j);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[r] = p[576 * r];
+    apply_tw8<S>(a, wn);
+    dft<8, S>(a);
+  }
+}
+
 }  // namespace gt

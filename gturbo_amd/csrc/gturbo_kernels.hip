@@ -441,22 +441,41 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   const unsigned r0 = yrow_off(y0, a.ntiles, BT), r1 = r0 + BT;  // rows y0, y0+1
   const unsigned fq = 0, fu = fstride, fv = 2 * fstride;
 
+  // 4096-point lines of a one-pair workgroup use the in-place transforms:
+  // the inverse ones leave x out of order, which the pointwise products do
+  // not mind, and the forward ones restore natural order for split_store.
+  constexpr bool IP = LOGN == 12 && NL == 1 && TWPF;
+  auto inv = [&](double2* v, auto bar) {
+    if constexpr (IP) fft_dif<+1, decltype(bar)::value>(v, buf, j, tw);
+    else fft_reg<LOGN, +1, SCHED, TWPF>(v, buf, j, tw);
+  };
+  auto fwd = [&](double2* v) {
+    if constexpr (IP) {
+      fft_dit<-1>(v, buf, j, tw);
+      __syncthreads();  // every wave's reads done before split_store writes
+    } else {
+      fft_reg<LOGN, -1, SCHED, TWPF>(v, buf, j, tw);
+    }
+  };
+  using first = std::false_type;  // no LDS reader to wait for
+  using after_dif = std::true_type;
+
   double2 z[kE], qq[kE];
   build_two<LOGN, TR, BT>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
-  fft_reg<LOGN, +1, SCHED, TWPF>(z, buf, j, tw);
+  inv(z, first{});
 #pragma unroll
   for (int r = 0; r < kE; ++r) qq[r] = z[r];                   // q0 + i q1
   build_two<LOGN, TR, BT>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
-  fft_reg<LOGN, +1, SCHED, TWPF>(z, buf, j, tw);
+  inv(z, after_dif{});
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1, SCHED, TWPF>(z, buf, j, tw);                   // (q v)0 + i (q v)1
+  fwd(z);                                                      // (q v)0 + i (q v)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, fstride + r0, fstride + r1, kxm);
   build_two<LOGN, TR, BT>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
-  fft_reg<LOGN, +1, SCHED, TWPF>(z, buf, j, tw);
+  inv(z, first{});  // split_store ends with a barrier
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
-  fft_reg<LOGN, -1, SCHED, TWPF>(z, buf, j, tw);                   // (q u)0 + i (q u)1
+  fwd(z);                                                      // (q u)0 + i (q u)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, r0, r1, kxm);
 }
 
@@ -478,7 +497,7 @@ __global__ void __launch_bounds__((1 << LOGN) / kE, 4)
     r_pass_pair(PassArgs a) {
   // Twiddles from memory: these workgroups are short-lived (one row pair),
   // and staging the table in LDS measured slower here.
-  __shared__ double2 lds[Line<LOGN>::PADDED];
+  __shared__ double2 lds[LOGN == 12 ? kIpLine : Line<LOGN>::PADDED];
   pair_body<LOGN, TR, bt_of(LOGN), 1, 0, true>(a, xcd_remap(blockIdx.x, gridDim.x),
                                                threadIdx.x, lds, a.tw);
 }
