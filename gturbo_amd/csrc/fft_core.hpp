@@ -294,31 +294,34 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// t[r] *= W^(k r) (conjugated for S > 0), from w1 = W^k.
+// t[r] *= W^(k r) (conjugated for S > 0), from w1 = W^k; each power is
+// applied as soon as it is formed (at most four live at a time).
 template <int S>
 __device__ __forceinline__ void apply_tw8(double2* t, double2 w1) {
-  double2 w[8];
-  w[1] = w1;
-  w[2] = cmul(w[1], w[1]);
-  w[3] = cmul(w[1], w[2]);
-  w[4] = cmul(w[2], w[2]);
-  w[5] = cmul(w[1], w[4]);
-  w[6] = cmul(w[2], w[4]);
-  w[7] = cmul(w[3], w[4]);
-#pragma unroll
-  for (int r = 1; r < 8; ++r) t[r] = S < 0 ? cmul(t[r], w[r]) : cmulc(t[r], w[r]);
+  auto ap = [](double2& x, double2 w) { x = S < 0 ? cmul(x, w) : cmulc(x, w); };
+  ap(t[1], w1);
+  const double2 w2 = cmul(w1, w1);
+  ap(t[2], w2);
+  const double2 w3 = cmul(w1, w2);
+  ap(t[3], w3);
+  const double2 w4 = cmul(w2, w2);
+  ap(t[4], w4);
+  ap(t[5], cmul(w1, w4));
+  ap(t[6], cmul(w2, w4));
+  ap(t[7], cmul(w3, w4));
 }
 
-// BAR: start with a workgroup barrier (another wave may still read LDS).
-template <int S, bool BAR>
+// bar: barrier before the first LDS write (another wave may still read
+// LDS); a uniform run-time flag, so that a loop holds one copy of the code.
+template <int S>
 __device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
-                                        const double2* __restrict__ tw) {
+                                        const double2* __restrict__ tw, bool bar) {
   asm volatile("" : "+v"(j));
   double2 wn = tw[j];  // pass 1: W^(j r)
   dft<8, S>(a);
   apply_tw8<S>(a, wn);
   wn = tw[8 * (j & 63)];  // pass 2, in flight across the barrier
-  if constexpr (BAR) __syncthreads();
+  if (bar) __syncthreads();
   {
     double2* p = buf + ip_pad(j);
 #pragma unroll
@@ -357,12 +360,15 @@ __device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
 // Input as fft_dif leaves it; the thread's own LDS run is free to write
 // (its last reader was this wave).  Ends after reads of every wave's
 // elements: a barrier must precede the next LDS write.
-template <int S>
+// BAR: start with a workgroup barrier (another wave may still read this
+// wave's run of LDS, e.g. after a preceding fft_dit).
+template <int S, bool BAR = false>
 __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
                                         const double2* __restrict__ tw) {
   asm volatile("" : "+v"(j));
   double2 wn = tw[64 * (j & 7)];
   dft<8, S>(a);  // pass 4, stride 1
+  if constexpr (BAR) __syncthreads();
   {
     double2* p = buf + 9 * j;
 #pragma unroll

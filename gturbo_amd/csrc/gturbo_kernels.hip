@@ -116,16 +116,29 @@ __device__ __forceinline__ void line_map(int tid, int& line, int& j) {
 
 // One tile of BT kx-columns; lds holds BT padded lines.  SCHED is the radix
 // order of the transforms (fft_reg).
-template <int LOGN, int MODE, int AK, int TR, int BT, int SCHED>
+// IP (4096-point lines only): the y buffers hold the rows of a column in
+// the order the in-place transforms use (slot j + 512 r holds physical row
+// D(j) + 512 r, D the octal digit reversal of j); the forward transforms
+// (fft_dit) take the loaded slots and return natural ky order, the inverse
+// ones (fft_dif) return slot order for the stores.  Each line is eight
+// whole waves: line b = wave mod 2, j = 64 (wave / 2) + lane.  The x pass
+// transforms rows, whichever rows a slot pair holds.
+template <int LOGN, int MODE, int AK, int TR, int BT, int SCHED, bool IP = false>
 __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
                                        double2* lds, const double2* tw) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
-  constexpr int LINE = Line<LOGN>::PADDED;
+  constexpr int LINE = IP ? kIpLine : Line<LOGN>::PADDED;
+  static_assert(!IP || (LOGN == 12 && BT == 2), "in-place y pass: 4096-point, two lines");
   constexpr bool FWD = MODE != C_START;
   constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
                        MODE == C_RK4_3 || MODE == C_RK2_1;
   int b, j;
-  line_map<TPL, BT>(tid, b, j);
+  if constexpr (IP) {
+    b = (tid >> 6) & 1;
+    j = ((tid >> 7) << 6) | (tid & 63);
+  } else {
+    line_map<TPL, BT>(tid, b, j);
+  }
   const int c = tile * BT + b;
   const Rect rc = a.rc;
   const bool colok = c <= rc.kx;
@@ -143,13 +156,15 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
     const double2* srcA = a.y2;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcA[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
-    fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
+    if constexpr (IP) fft_dit<-1, false>(z, buf, j, tw);
+    else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
 #pragma unroll
     for (int r = 0; r < kE; ++r) A[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
     const double2* srcB = a.y2 + (size_t)a.ntiles * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcB[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
-    fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
+    if constexpr (IP) fft_dit<-1, true>(z, buf, j, tw);
+    else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
     int je = j;
     // The epilogue's indices (and so its loads) depend on the transform's
     // output: nothing of the epilogue is scheduled inside the last pass.
@@ -275,7 +290,12 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
         }
         z[r] = v;
       }
-      fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
+      if constexpr (IP) {
+        // a barrier first unless nothing has used LDS since the table staging
+        fft_dif<+1>(z, buf, j, tw, FWD || f > 0);
+      } else {
+        fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
+      }
       double2* dst = a.y1 + (size_t)f * a.ntiles * N * BT;
       const unsigned colo = ycol_off(c, BT);
 #pragma unroll
@@ -288,10 +308,14 @@ template <int LOGN, int MODE, int AK, int TR>
 __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 1)
     c_pass(PassArgs a) {
   constexpr int BT = bt_of(LOGN);
-  __shared__ double2 lds[BT * Line<LOGN>::PADDED];
+  // Row slots in transform order (in-place transforms) for alpha = 2 on the
+  // 2/3 rectangle; every y pass of a run has the same AK and TR, so the
+  // buffers' row order is the same in all of them.
+  constexpr bool IP = LOGN == 12 && AK == 1 && TR == 1;
+  __shared__ double2 lds[BT * (IP ? kIpLine : Line<LOGN>::PADDED)];
   __shared__ double2 twl[tw_lds_size<LOGN>()];
   const double2* tw = stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
-  c_body<LOGN, MODE, AK, TR, BT, 0>(a, blockIdx.x, threadIdx.x, lds, tw);
+  c_body<LOGN, MODE, AK, TR, BT, 0, IP>(a, blockIdx.x, threadIdx.x, lds, tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,7 +470,7 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   // not mind, and the forward ones restore natural order for split_store.
   constexpr bool IP = LOGN == 12 && NL == 1 && TWPF;
   auto inv = [&](double2* v, auto bar) {
-    if constexpr (IP) fft_dif<+1, decltype(bar)::value>(v, buf, j, tw);
+    if constexpr (IP) fft_dif<+1>(v, buf, j, tw, decltype(bar)::value);
     else fft_reg<LOGN, +1, SCHED, TWPF>(v, buf, j, tw);
   };
   auto fwd = [&](double2* v) {
