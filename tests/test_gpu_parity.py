@@ -246,6 +246,43 @@ def test_headline_workload_steps_match_oracle(oracle_mod, rko):
     assert err < 1e-12
 
 
+@pytest.mark.parametrize("alpv,truncate", [(2.0, 1), (1.0, 1), (2.0, 0)])
+def test_4096_nlt_and_steps_on_each_transform_path(oracle_mod, alpv, truncate):
+    """At 4096^2 the x pass always runs the in-place transforms, and the y
+    passes do for alpha = 2 on the 2/3 rectangle (rows kept in transform
+    order between the passes) and otherwise run the Stockham ones: nlt, one
+    RK4 step and one RK2 step on each combination."""
+    n = 4096
+    for rko in (4, 2):
+        p = params_for(n, alpv=alpv, truncate=truncate, rko=rko)
+        s, o = _pair(oracle_mod, p)
+        z = random_spectral(n, 31 + rko, truncated=bool(truncate))
+        zg, ug, vg = z.copy(), s.field(), s.field()
+        _present(s, zg, ug, vg)
+        print("4096 alpha", alpv, "truncate", truncate, "rko", rko, flush=True)
+        if rko == 4:  # nlt once, on its own
+            zn = z.copy()
+            zng = zn.copy()
+            _present(s, zng)
+            s.nlt(zng, ug, vg)
+            s.update_host(zng)
+            o.nlt(zn)
+            assert rel_l2(zng, zn) < 1e-12
+        scratch = [s.field() for _ in range(4)]
+        _present(s, *scratch)
+        if rko == 4:
+            s.step4(zg, *scratch)
+            o.step4(z)
+        else:
+            s.step(zg, scratch[0], scratch[2], scratch[3])
+            o.step(z)
+        s.update_host(zg)
+        err = rel_l2(zg, z)
+        print(f"  step rel L2 = {err:.3e}", flush=True)
+        assert err < 1e-12
+        s.close()
+
+
 def test_sqg1024_workload_with_diagnostics_matches_oracle(oracle_mod):
     """The 1024^2 SQG workload (alpha=1, RK4, dt=2e-4, nu=1e-22 k^8,
     mu=0.05, ring (31,33], seed-1 initial field, spectra and fluxes every 100
