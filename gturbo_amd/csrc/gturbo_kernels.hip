@@ -469,8 +469,10 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   // the inverse ones leave x out of order, which the pointwise products do
   // not mind, and the forward ones restore natural order for split_store.
   constexpr bool IP = LOGN == 12 && NL == 1 && TWPF;
-  auto inv = [&](double2* v, auto bar) {
-    if constexpr (IP) fft_dif<+1>(v, buf, j, tw, decltype(bar)::value);
+  // bar: another wave may still read LDS (ignored by the Stockham transforms,
+  // which synchronise on every transpose)
+  auto inv = [&](double2* v, bool bar) {
+    if constexpr (IP) fft_dif<+1>(v, buf, j, tw, bar);
     else fft_reg<LOGN, +1, SCHED, TWPF>(v, buf, j, tw);
   };
   auto fwd = [&](double2* v) {
@@ -481,22 +483,19 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
       fft_reg<LOGN, -1, SCHED, TWPF>(v, buf, j, tw);
     }
   };
-  using first = std::false_type;  // no LDS reader to wait for
-  using after_dif = std::true_type;
-
   double2 z[kE], qq[kE];
   build_two<LOGN, TR, BT>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
-  inv(z, first{});
+  inv(z, false);  // no LDS reader yet
 #pragma unroll
   for (int r = 0; r < kE; ++r) qq[r] = z[r];                   // q0 + i q1
   build_two<LOGN, TR, BT>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
-  inv(z, after_dif{});
+  inv(z, true);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
   fwd(z);                                                      // (q v)0 + i (q v)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, fstride + r0, fstride + r1, kxm);
   build_two<LOGN, TR, BT>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
-  inv(z, first{});  // split_store ends with a barrier
+  inv(z, false);  // split_store ends with a barrier
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
   fwd(z);                                                      // (q u)0 + i (q u)1
