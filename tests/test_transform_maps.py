@@ -1,0 +1,143 @@
+"""CPU checks of the index algebra the HIP transforms rely on (no GPU):
+
+* the padded LDS index splits into a per-thread base plus compile-time
+  offsets for every Stockham pass and for the mirror read of the
+  half-spectrum split (`lpad_off`, `mirror_at` in the kernels);
+* the in-place radix-8 transforms of a 4096-point line (`fft_dif`,
+  `fft_dit`): a step-by-step simulation of their passes, with the same LDS
+  positions, twiddles and thread/register ownership, reproduces the DFT,
+  DIF leaving X[D(j) + 512 r] in thread j, register r, and DIT taking that
+  order back to natural order; passes 2-4 touch only the wave's own run.
+"""
+import numpy as np
+import pytest
+
+
+def lpad(e):
+    return e + (e >> 4)
+
+
+def lpad_off(c):
+    return c + c // 16
+
+
+def _passes(logn, sched):
+    rl, p8 = 1 << (logn % 3), logn // 3
+    if sched == 0 or rl == 1:
+        return [(8, 8 ** i) for i in range(p8)] + ([(rl, 8 ** p8)] if rl > 1 else [])
+    return [(rl, 1)] + [(8, rl * 8 ** i) for i in range(p8)]
+
+
+@pytest.mark.parametrize("logn", range(4, 14))
+def test_stockham_lds_offsets_split(logn):
+    n, tpl = 1 << logn, (1 << logn) // 8
+    for sched in (0, 1):
+        for radix, ns in _passes(logn, sched):
+            nb = 8 // radix
+            for j in range(tpl):
+                for b in range(nb):
+                    jj = j + tpl * b
+                    base = (jj // ns) * ns * radix + jj % ns
+                    for r in range(radix):
+                        assert lpad(base + r * ns) == lpad(base) + lpad_off(r * ns)
+    for j in range(tpl):
+        for r in range(8):
+            assert lpad(j + tpl * r) == lpad(j) + lpad_off(tpl * r)
+        if tpl % 16 == 0:  # mirror_at's split form
+            pm = lpad(tpl - j)
+            for r in range(5):
+                want = lpad((n - j - tpl * r) & (n - 1))
+                got = (0 if j == 0 else pm + lpad_off(7 * tpl)) if r == 0 else \
+                    pm + lpad_off(tpl * (7 - r))
+                assert got == want
+
+
+N, T = 4096, 512
+
+
+def ip_pad(e):
+    return e + (e >> 3)
+
+
+def D(j):
+    return 64 * (j & 7) + 8 * ((j >> 3) & 7) + (j >> 6)
+
+
+def _dft8(t, s):
+    k = np.arange(8)
+    return np.exp(s * 2j * np.pi * np.outer(k, k) / 8) @ t
+
+
+def _tw(t, k, s):
+    return t * np.exp(s * 2j * np.pi * k * np.arange(8) / N)
+
+
+def _runs(j):
+    """LDS positions of thread j's runs in passes 2, 3, 4 (element indices)."""
+    b2 = (j >> 6) * 512 + (j & 63)
+    b3 = (j >> 3) * 64 + (j & 7)
+    return ([ip_pad(b2) + 72 * r for r in range(8)], [ip_pad(b3) + 9 * r for r in range(8)],
+            [9 * j + r for r in range(8)])
+
+
+def _dif(a, s):
+    buf = np.zeros(N + N // 8, complex)
+    a = a.copy()
+    for j in range(T):
+        a[j] = _tw(_dft8(a[j], s), j, s)
+        for r in range(8):
+            buf[ip_pad(j) + 576 * r] = a[j][r]
+    for j in range(T):
+        p2 = _runs(j)[0]
+        buf[p2] = _tw(_dft8(buf[p2], s), 8 * (j & 63), s)
+    for j in range(T):
+        p3 = _runs(j)[1]
+        buf[p3] = _tw(_dft8(buf[p3], s), 64 * (j & 7), s)
+    for j in range(T):
+        a[j] = _dft8(buf[_runs(j)[2]], s)
+    return a
+
+
+def _dit(a, s):
+    buf = np.zeros(N + N // 8, complex)
+    a = a.copy()
+    for j in range(T):
+        buf[_runs(j)[2]] = _dft8(a[j], s)
+    for j in range(T):
+        p3 = _runs(j)[1]
+        buf[p3] = _dft8(_tw(buf[p3], 64 * (j & 7), s), s)
+    for j in range(T):
+        p2 = _runs(j)[0]
+        buf[p2] = _dft8(_tw(buf[p2], 8 * (j & 63), s), s)
+    for j in range(T):
+        a[j] = _dft8(_tw(buf[[ip_pad(j) + 576 * r for r in range(8)]], j, s), s)
+    return a
+
+
+@pytest.mark.parametrize("s", [+1, -1])
+def test_in_place_transforms_reproduce_the_dft(s):
+    x = np.random.default_rng(3).standard_normal(N) + 1j * np.random.default_rng(4).standard_normal(N)
+    ref = np.fft.ifft(x) * N if s > 0 else np.fft.fft(x)
+    nat = np.array([[x[j + 512 * r] for r in range(8)] for j in range(T)])
+    out = _dif(nat, s)
+    want = np.array([[ref[D(j) + 512 * r] for r in range(8)] for j in range(T)])
+    assert np.abs(out - want).max() < 1e-9 * np.abs(ref).max()
+    perm = np.array([[x[D(j) + 512 * r] for r in range(8)] for j in range(T)])
+    out2 = _dit(perm, s)
+    want2 = np.array([[ref[j + 512 * r] for r in range(8)] for j in range(T)])
+    assert np.abs(out2 - want2).max() < 1e-9 * np.abs(ref).max()
+
+
+def test_in_place_layout_and_ownership():
+    pads = [ip_pad(e) for e in range(N)]
+    assert len(set(pads)) == N and max(pads) < N + N // 8
+    assert all(D(D(j)) == j for j in range(T))  # digit reversal is an involution
+    for j in range(T):
+        w = j >> 6
+        p2, p3, p4 = _runs(j)
+        # pass 2: the wave's own 512 elements; passes 3, 4: one 64-element run
+        # written and read by the same eight lanes
+        lo = ip_pad(512 * w)
+        assert all(lo <= p < lo + 576 for p in p2)
+        run = ip_pad(64 * (j >> 3))
+        assert all(run <= p < run + 72 for p in p3 + p4)
