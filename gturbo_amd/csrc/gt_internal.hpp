@@ -94,15 +94,12 @@ enum CMode : int {
 
 // Column-tile width of the y pass (BT), y rows per workgroup of the x pass,
 // and row pairs per workgroup of the reference-layout transforms, per
-// log2 N.  A line is N/8 threads and (N + N/16) double2 of LDS.
-__host__ __device__ constexpr int bt_of(int logn) {
-  return logn <= 6 ? 64 : logn == 7 ? 32 : logn == 8 ? 16 : logn == 9 ? 8
-       : logn == 10 ? 4 : logn <= 12 ? 2 : 1;
-}
-__host__ __device__ constexpr int rows_of(int logn) {
-  return logn <= 6 ? (1 << logn) : logn == 7 ? 32 : logn == 8 ? 16
-       : logn == 9 ? 8 : logn == 10 ? 4 : logn <= 12 ? 2 : 1;
-}
+// log2 N.  A line is N/8 threads and (N + N/16) double2 of LDS.  Two
+// columns (rows) per workgroup up to N = 4096: a 4-row block of two columns
+// is one 128-byte line, and small grids get as many workgroups as there
+// are column pairs (wider tiles left 256^2 with six y workgroups).
+__host__ __device__ constexpr int bt_of(int logn) { return logn <= 12 ? 2 : 1; }
+__host__ __device__ constexpr int rows_of(int logn) { return logn <= 12 ? 2 : 1; }
 __host__ __device__ constexpr int rp_of(int logn) {
   return logn <= 6 ? (1 << (logn - 1)) : logn == 7 ? 32 : logn == 8 ? 16
        : logn == 9 ? 8 : logn == 10 ? 4 : logn == 11 ? 2 : 1;
