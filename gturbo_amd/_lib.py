@@ -47,6 +47,7 @@ SIGNATURES = {
     "gt_glob": (C.c_int, [_P, _P, _P]),
     "gt_shell_index": (C.c_int, [_P, _P]),
     "gt_timing_enable": (C.c_int, [_P, _I32]),
+    "gt_timing_classes": (C.c_int, [_P, _I32]),
     "gt_timing_read": (C.c_int, [_P, _P, _P]),
     "gt_adj_forward": (C.c_int, [_P, _P, _P, _I32, _I32, C.POINTER(_I32)]),
     "gt_adj_backward": (C.c_int, [_P, _P, _P]),

@@ -103,6 +103,7 @@ struct gt_ctx {
 
   // timing
   bool timing = false;
+  unsigned timing_mask = ~0u;  // classes timed (bit 1 << GT_K_*)
   std::vector<gtr::TimedLaunch> tl;
   std::vector<hipEvent_t> ev_pool;
 

@@ -70,14 +70,14 @@ hipEvent_t get_event(gt_ctx* c) {
 }
 
 Timer::Timer(gt_ctx* c_, int cls_) : c(c_), cls(cls_) {
-  if (c->timing) {
+  if (c->timing && (c->timing_mask >> cls & 1u)) {
     a = get_event(c);
     if (a) hipEventRecord(a, c->stream);
   }
 }
 
 Timer::~Timer() {
-  if (c->timing && a) {
+  if (a) {
     hipEvent_t b = get_event(c);
     if (b) {
       hipEventRecord(b, c->stream);
@@ -978,6 +978,12 @@ int gt_shell_index(gt_ctx* c, int32_t* ib) {
 int gt_timing_enable(gt_ctx* c, int32_t on) {
   if (!c) return fail(GT_ERR_ARG, "null context");
   c->timing = on != 0;
+  return GT_OK;
+}
+
+int gt_timing_classes(gt_ctx* c, int32_t mask) {
+  if (!c) return fail(GT_ERR_ARG, "null context");
+  c->timing_mask = (unsigned)mask;
   return GT_OK;
 }
 

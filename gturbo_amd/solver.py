@@ -191,7 +191,11 @@ class Solver:
         return ib
 
     # ---- timing ---------------------------------------------------------------
-    def timing_enable(self, on=True):
+    def timing_enable(self, on=True, classes=None):
+        """Event-time kernel launches on the stream; classes: names from
+        K_NAMES to time (default: all)."""
+        mask = -1 if classes is None else sum(1 << K_NAMES.index(k) for k in classes)
+        check(self.L.gt_timing_classes(self.h, mask))
         check(self.L.gt_timing_enable(self.h, 1 if on else 0))
 
     def timing_read(self):

@@ -142,6 +142,11 @@ int gt_shell_index(gt_ctx *ctx, int32_t *ib);
 #define GT_K_OTHER 5
 #define GT_K_NCLASS 6
 int gt_timing_enable(gt_ctx *ctx, int32_t on);
+/* Which classes gt_timing_enable times: bit (1 << GT_K_*) set = timed
+ * (default: all).  Each timed launch records two events on the stream;
+ * timing only the class of interest keeps the others' launches back to
+ * back. */
+int gt_timing_classes(gt_ctx *ctx, int32_t mask);
 /* Sums of event-timed kernel durations per class since the last read (ms)
  * and launch counts; resets the sums. */
 int gt_timing_read(gt_ctx *ctx, double *ms, int32_t *count);
