@@ -42,7 +42,7 @@ def _tlm(o, w0, dw0, nsteps, seed):
 
 @pytest.mark.parametrize("n,alpv,rko,nsteps,kchk", [
     (64, 2.0, 4, 7, 3), (64, 1.0, 4, 6, 6), (128, 0.5, 4, 5, 2), (64, 1.0, 2, 8, 3),
-    (256, 1.0, 4, 4, 2)])
+    (256, 1.0, 4, 4, 2), (4096, 2.0, 4, 2, 1)])
 def test_adjoint_dot_product_with_oracle_tangent(oracle_mod, n, alpv, rko, nsteps, kchk):
     p = params_for(n, alpv=alpv, rko=rko, famp=1.0)
     w0 = ic.physical_field(p, InitialSpectrum(seed=11, p=4.0, k0=n / 16.0, energy=0.5))
