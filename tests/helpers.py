@@ -5,7 +5,8 @@ from gturbo_amd.params import Params, TWO_PI
 
 
 def params_for(n, alpv=2.0, rko=4, truncate=1, nu=None, mu=0.1, alpnu=4.0, alpmu=0.0,
-               dt=None, famp=1.0, ring=None, rtrunc=0.6666666666666666667, xlx=TWO_PI):
+               dt=None, famp=1.0, ring=None, rtrunc=0.6666666666666666667, xlx=TWO_PI,
+               xly=None):
     if nu is None:
         nu = (n / 256.0) ** -8 * 1e-16 if alpnu == 4.0 else 1e-3
     if dt is None:
@@ -13,7 +14,7 @@ def params_for(n, alpv=2.0, rko=4, truncate=1, nu=None, mu=0.1, alpnu=4.0, alpmu
     if ring is None:
         kf = max(n // 16, 2)
         ring = (kf - 1.0, kf + 1.0)
-    return Params(n=n, xlx=xlx, xly=xlx, nu=nu, alpnu=alpnu, mu=mu, alpmu=alpmu, alpv=alpv,
+    return Params(n=n, xlx=xlx, xly=xlx if xly is None else xly, nu=nu, alpnu=alpnu, mu=mu, alpmu=alpmu, alpv=alpv,
                   dt=dt, rko=rko, npas=10, iout=10**9, imix=10**9, rtrunc=rtrunc,
                   truncate=truncate, famp=famp, k1f=ring[0], k2f=ring[1])
 

@@ -207,6 +207,52 @@ def test_diagnostics_match_oracle(oracle_mod, n, alpv):
     assert np.allclose(g, og, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("n,xlx,xly,rtrunc,alpmu,alpnu,alpv,steps", [
+    (128, 4 * np.pi, 4 * np.pi, 2 / 3, 0.0, 4.0, 2.0, 3),   # larger box: dk = 1/2
+    (128, 2 * np.pi, np.pi, 2 / 3, 0.0, 4.0, 1.0, 3),       # anisotropic box
+    (256, 2 * np.pi, 2 * np.pi, 0.5, -1.0, 2.0, 2.0, 3),    # smaller radius, drag mu k^-2
+    (256, 2 * np.pi, 2 * np.pi, 0.9, 0.0, 4.0, 0.5, 3),     # past 2/3: full rectangle
+    (4096, 2 * np.pi, 2 * np.pi, 0.9, 0.0, 4.0, 2.0, 1),    # the same at 4096
+])
+def test_other_parameters_steps_and_diagnostics(oracle_mod, n, xlx, xly, rtrunc, alpmu,
+                                                alpnu, alpv, steps):
+    """Domain sizes other than 2 pi (square and not), truncation radii
+    other than 2/3, a drag exponent other than 0 and dissipation k^4: RK4
+    steps with forcing, then spectrum, fluxes and glob, against the oracle."""
+    p = params_for(n, alpv=alpv, alpnu=alpnu, alpmu=alpmu, rtrunc=rtrunc, xlx=xlx, xly=xly)
+    s, o = _pair(oracle_mod, p)
+    assert s.nf == o.nf and s.nf > 0
+    z = random_spectral(n, 90 + n, truncated=rtrunc <= 2 / 3 + 1e-12)
+    o.trunc(z)
+    zg = z.copy()
+    scratch = [s.field() for _ in range(4)]
+    _present(s, zg, *scratch)
+    seed_g = seed_o = 481929
+    print(n, xlx, xly, rtrunc, alpmu, flush=True)
+    for _ in range(steps):
+        s.step4(zg, *scratch)
+        seed_g = s.forcing(zg, seed_g)
+        o.step4(z)
+        seed_o = o.forcing(z, seed_o)
+    s.update_host(zg)
+    assert seed_g == seed_o
+    err = rel_l2(zg, z)
+    print(f"  {steps} steps rel L2 = {err:.3e}", flush=True)
+    assert err < 1e-12
+    znl = z.copy()
+    o.nlt(znl)
+    ng = znl.copy()
+    _present(s, ng)
+    sp, sz = s.spectrum(zg)
+    osp, osz = o.spectrum(z)
+    assert rel_l2(sp, osp) < 1e-12 and rel_l2(sz, osz) < 1e-12
+    fle, flz = s.fluxes(zg, ng)
+    ofle, oflz = o.fluxes(z, znl)
+    assert np.abs(fle - ofle).max() / (np.abs(ofle).max() + 1e-300) < 1e-9
+    assert np.abs(flz - oflz).max() / (np.abs(oflz).max() + 1e-300) < 1e-9
+    assert np.allclose(s.glob(zg), o.glob(z), rtol=1e-12, atol=0)
+
+
 def test_run_euler256_matches_oracle(oracle_mod):
     """The 256^2 Euler workload (alpha=2, RK4, dt=1e-3, nu=1e-16 k^8, mu=0.1,
     ring (15,17]) over its full 1000 steps from the same float32 initial field
