@@ -60,6 +60,7 @@ SIGNATURES = {
     "gt_ens_schedule": (C.c_int, [_P, _P, _I32]),
     "gt_ens_tune": (C.c_int, [_P, _I32, _P, C.POINTER(C.c_double)]),
     "gt_ens_run": (C.c_int, [_P, _I32]),
+    "gt_ens_advance": (C.c_int, [_P, _I32, _I32]),
     "gt_ens_synchronize": (C.c_int, [_P]),
 }
 

@@ -64,6 +64,7 @@ struct gt_ens {
   hipEvent_t ph_ev[2] = {};
   bool ph_used[2] = {};
   unsigned nchunk = 0;
+  bool pending = false;  // the last step's forcing is held back (gt_ens_advance)
   int* zero_host = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -203,6 +204,16 @@ int capture(gt_ens* e) {
     return fail(GT_ERR_HIP, "graph capture: %s", hipGetErrorString(ee));
   e->graph = g;
   HIPCHK(hipGraphInstantiate(&e->gexec, e->graph, nullptr, nullptr, 0));
+  return GT_OK;
+}
+
+// The held-back forcing of gt_ens_advance, on the step counter and phase
+// buffer it was drawn with.
+int apply_pending(gt_ens* e) {
+  if (!e->pending) return GT_OK;
+  HIPCHK(gt::launch_ens_forcing(e->fdesc, e->nmem, e->ftotal, e->step, e->stream));
+  HIPCHK(gt::launch_ens_tick(e->step, e->stream));
+  e->pending = false;
   return GT_OK;
 }
 
@@ -463,14 +474,23 @@ int gt_ens_schedule(gt_ens* e, int32_t* cfg, int32_t set) {
 
 int gt_ens_run(gt_ens* e, int32_t nsteps) {
   GT_RANGE();
+  return gt_ens_advance(e, nsteps, 0);
+}
+
+int gt_ens_advance(gt_ens* e, int32_t nsteps, int32_t hold) {
+  GT_RANGE();
   if (!e) return fail(GT_ERR_ARG, "null ensemble");
   if (nsteps < 0) return fail(GT_ERR_ARG, "negative step count");
+  CHK(apply_pending(e));
   if (nsteps == 0) return GT_OK;
   if (!e->gexec) CHK(capture(e));
   const int gs = e->cfg[5];
   const int cap = std::max(gs, kPipe / gs * gs);
   for (int left = nsteps; left > 0;) {
     const int ch = std::min(left, cap);
+    // the held step is the chunk's last: its phases are drawn and uploaded
+    // with the chunk, and the step counter points at them when it is applied
+    const int full = (hold && ch == left) ? ch - 1 : ch;
     const int b = (int)(e->nchunk++ % 2);
     if (e->ph_used[b]) HIPCHK(hipEventSynchronize(e->ph_ev[b]));  // host buffer free
     draw_phases(e, b, ch);
@@ -482,8 +502,12 @@ int gt_ens_run(gt_ens* e, int32_t nsteps) {
     HIPCHK(hipMemcpyAsync(e->step, e->zero_host, sizeof(int), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipEventRecord(e->ph_ev[b], e->stream));
     e->ph_used[b] = true;
-    for (int i = 0; i < ch / gs; ++i) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
-    for (int i = 0; i < ch % gs; ++i) CHK(launch_step(e, true));
+    for (int i = 0; i < full / gs; ++i) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+    for (int i = 0; i < full % gs; ++i) CHK(launch_step(e, true));
+    if (full < ch) {
+      CHK(launch_step(e, false));
+      e->pending = true;
+    }
     left -= ch;
   }
   return GT_OK;
@@ -499,6 +523,7 @@ int gt_ens_tune(gt_ens* e, int32_t steps, int32_t* cfg_out, double* ms_out) {
   GT_RANGE();
   if (!e) return fail(GT_ERR_ARG, "null ensemble");
   if (steps < 1) return fail(GT_ERR_ARG, "steps must be positive");
+  CHK(apply_pending(e));
   HIPCHK(hipStreamSynchronize(e->stream));
   std::vector<double2*> backup(e->nmem, nullptr);
   int rc = GT_OK;

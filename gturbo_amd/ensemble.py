@@ -111,6 +111,13 @@ class Ensemble:
         """nsteps x (RK step, then forcing) for every member; asynchronous."""
         check(self.L.gt_ens_run(self.h, int(nsteps)))
 
+    def advance(self, nsteps: int, hold: bool = False):
+        """As run; with hold the last step's forcing waits for the next
+        advance/run (advance(0) applies it alone), so that the fields in
+        between show the state the reference's driver takes diagnostics and
+        snapshots of."""
+        check(self.L.gt_ens_advance(self.h, int(nsteps), 1 if hold else 0))
+
     def synchronize(self):
         check(self.L.gt_ens_synchronize(self.h))
 

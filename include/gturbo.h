@@ -219,6 +219,13 @@ int gt_ens_schedule(gt_ens *ens, int32_t *cfg, int32_t set);
 int gt_ens_tune(gt_ens *ens, int32_t steps, int32_t *cfg, double *ms_per_step);
 /* nsteps x (RK step of every member, then its forcing).  Asynchronous. */
 int gt_ens_run(gt_ens *ens, int32_t nsteps);
+/* As gt_ens_run; with hold != 0 the forcing of the last of the nsteps steps
+ * is held back, so that the fields show the state after that step's RK
+ * update and before its forcing -- where the reference's driver takes its
+ * diagnostics and snapshots (src/main.f90:97-136).  A held forcing is applied
+ * first by the next gt_ens_advance / gt_ens_run (nsteps = 0 applies it
+ * alone) or gt_ens_tune. */
+int gt_ens_advance(gt_ens *ens, int32_t nsteps, int32_t hold);
 int gt_ens_synchronize(gt_ens *ens);
 
 #ifdef __cplusplus
