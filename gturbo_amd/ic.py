@@ -11,8 +11,14 @@ reference reads (src/read.f90:10-19).
 """
 from __future__ import annotations
 
+import argparse
+import os
+import shutil
+import sys
+
 import numpy as np
 
+from . import io
 from .params import InitialSpectrum, Params
 
 
@@ -61,3 +67,54 @@ def physical_field(p: Params, ic: InitialSpectrum) -> np.ndarray:
     q = spectral_field(p, ic)
     w = np.fft.irfft2(q, s=(p.n, p.n)) * (p.n * p.n)
     return w.astype(np.float32)
+
+
+def make_runs(params_path: str, n: int, copies: int, prefix: str = "", ic_seed: int = 1,
+              forcing_seed: int = 481929, p: float = 4.0, k0: float = 8.0,
+              energy: float = 0.5, same_forcing_seed: bool = False) -> list:
+    """What src/reset.sh does before submitting its jobs: run directories
+    {prefix}1run ... {prefix}{copies}run, each with a copy of params.dat, its
+    own initial field (seed ic_seed + k) in fields/w.001, files/seed.001 (the
+    forcing seed, forcing_seed + k unless same_forcing_seed) and curframe.dat
+    "1 0".  Returns the directories."""
+    par = Params.read(params_path, n)
+    dirs = []
+    for k in range(copies):
+        d = f"{prefix}{k + 1}run"
+        w = physical_field(par, InitialSpectrum(seed=ic_seed + k, p=p, k0=k0, energy=energy))
+        os.makedirs(os.path.join(d, "files"), exist_ok=True)
+        os.makedirs(os.path.join(d, "fields"), exist_ok=True)
+        dst = os.path.join(d, "params.dat")
+        if not (os.path.exists(dst) and os.path.samefile(params_path, dst)):
+            shutil.copyfile(params_path, dst)  # the file itself, as cp -r does
+        io.write_curframe(os.path.join(d, "curframe.dat"), 1, 0.0)
+        io.write_seed(os.path.join(d, "files", "seed.001"),
+                      forcing_seed + (0 if same_forcing_seed else k))
+        io.write_field(os.path.join(d, "fields", "w.001"), w)
+        dirs.append(d)
+    return dirs
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(
+        description="Lay out run directories with random initial fields (src/reset.sh with "
+                    "its gen_noise.py).")
+    ap.add_argument("params", help="the params.dat to copy into every directory")
+    ap.add_argument("--n", type=int, default=8192, help="grid size (the driver's argument)")
+    ap.add_argument("--copies", type=int, default=1)
+    ap.add_argument("--prefix", default="", help="directory name prefix ({prefix}1run, ...)")
+    ap.add_argument("--ic-seed", type=int, default=1, help="seed of the first directory's field")
+    ap.add_argument("--forcing-seed", type=int, default=481929, help="src/seed.0")
+    ap.add_argument("--same-forcing-seed", action="store_true")
+    ap.add_argument("--slope", type=float, default=4.0, help="p of k**p exp(-(k/k0)**2)")
+    ap.add_argument("--k0", type=float, default=8.0)
+    ap.add_argument("--energy", type=float, default=0.5)
+    a = ap.parse_args(argv)
+    for d in make_runs(a.params, a.n, a.copies, a.prefix, a.ic_seed, a.forcing_seed, a.slope,
+                       a.k0, a.energy, a.same_forcing_seed):
+        print(d)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

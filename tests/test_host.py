@@ -134,3 +134,36 @@ def test_library_without_gpu_fails_cleanly():
     from gturbo_amd.solver import Solver
     with pytest.raises(_lib.GturboError, match="error 5"):
         Solver(params.workloads()["euler256"].params)
+
+
+def test_make_runs_lays_out_run_directories(tmp_path, monkeypatch):
+    """gturbo_amd.ic.make_runs (src/reset.sh with its gen_noise.py): every
+    directory gets the params.dat, its own float32 field at the requested
+    energy, its forcing seed and curframe "1 0"."""
+    from gturbo_amd import ic, io
+    from gturbo_amd.params import Params
+    monkeypatch.chdir(tmp_path)
+    n = 32
+    Params(n=n).write("params.dat")
+    dirs = ic.make_runs("params.dat", n, 3, prefix="e", ic_seed=5, forcing_seed=100)
+    assert dirs == ["e1run", "e2run", "e3run"]
+    fields = []
+    for k, d in enumerate(dirs):
+        assert open(os.path.join(d, "params.dat")).read() == open("params.dat").read()
+        assert io.read_curframe(os.path.join(d, "curframe.dat")) == (1, 0.0)
+        assert io.read_seed(os.path.join(d, "files", "seed.001")) == 100 + k
+        w = io.read_field(os.path.join(d, "fields", "w.001"), n)
+        assert w.dtype == np.float32 and w.shape == (n, n)
+        # energy of the float32 field: sum fac |q|^2 / k^alpha with q = fft/N^2
+        p = Params.read("params.dat", n)
+        q = np.fft.rfft2(w.astype(np.float64)) / n ** 2
+        kx = np.arange(n // 2 + 1)
+        ky = np.fft.fftfreq(n, 1.0 / n)
+        k2 = kx[None, :] ** 2 + ky[:, None] ** 2
+        fac = np.ones(k2.shape)
+        fac[:, 0] = fac[:, -1] = 0.5
+        with np.errstate(divide="ignore"):
+            e = np.sum(np.where(k2 > 0, fac * np.abs(q) ** 2 / k2 ** (p.alpv / 2), 0.0))
+        assert abs(e - 0.5) < 1e-4
+        fields.append(w)
+    assert not np.array_equal(fields[0], fields[1])
