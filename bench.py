@@ -7,7 +7,8 @@ A step is what the reference's driver loop does per iteration when no
 diagnostics or output are due (src/main.f90:97-137): step4 + forcing, on a
 state resident in HBM.  W warm-up steps, then K timed steps bracketed by
 device synchronisation.  The roofline entry is the kernel class with the
-largest time per step, priced at its algorithmic bytes over its average
+largest time per step, priced at its algorithmic bytes (the path's fixed
+byte model, `model_bytes`) over its average
 duration measured live over the timed region with HIP events on the
 library's own stream (only that class is event-timed there: every timed
 launch records two events, which costs a step a few per cent).  The
@@ -42,10 +43,31 @@ METRIC = "RK4 grid-pt-steps/s at 4096² FP64; ensemble & adjoint vs plain loop"
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(n, rect, rko):
+def model_bytes(n, rko):
+    """Algorithmic HBM bytes per launch of each kernel class in the fixed byte
+    model of the path (A = 8 N^2 bytes, one pass over an FP64 half-spectrum
+    field): per nonlinear term 11A -- the inverse y transforms read q (1A)
+    and write q, u, v (3A), the x pass reads 3A and writes 2A, the forward
+    y transforms read 2A -- plus the RK4 epilogues 14.5A per step (stage 1
+    3.5A, stages 2 and 3 4.5A each, the final update 2A); RK2: 27A per step.
+    Distributed over the fused kernels: x pass 5A; y start 4A; y stage
+    2A + epilogue + 4A; y final 2A + epilogue.  RK4: 58.5A = 468 B per
+    grid point and step."""
+    a = 8.0 * n * n
+    if rko == 4:
+        stage = 2 * a + (3.5 + 4.5 + 4.5) / 3 * a + 4 * a  # stages 1-3 averaged
+        final = 2 * a + 2 * a
+    else:  # RK2 27A: y start 4A, 2 x 5A, stage 1 2A + 2.5A + 4A, final 2A + 2.5A
+        stage = 2 * a + 2.5 * a + 4 * a
+        final = 2 * a + 2.5 * a
+    return {"y_start": 4 * a, "y_stage": stage, "y_final": final, "x_pass": 5 * a}
+
+
+def compact_bytes(n, rect, rko):
     """HBM bytes each kernel class must move per launch on the compact layout
-    (a kept mode is 16 B; a y-buffer field holds N*kc complex values; emk is
-    8 B per kept mode)."""
+    the kernels use (a kept mode is 16 B; a y-buffer field holds N*kc complex
+    values; emk is 8 B per kept mode) -- what the measured traffic is checked
+    against."""
     kx, p, q = rect
     kc, ky = kx + 1, p + 1 + q
     Z = 16.0 * kc * ky          # one compact spectral field
@@ -301,7 +323,8 @@ def main():
 
     for _ in range(args.warmup):
         seed = one_step(seed)
-    ab = algorithmic_bytes(p.n, s.rect, p.rko)
+    ab = model_bytes(p.n, p.rko)
+    cb = compact_bytes(p.n, s.rect, p.rko)
 
     # Kernel breakdown: a sample of its own with every kernel class
     # event-timed (each timed launch records two events on the stream,
@@ -323,6 +346,8 @@ def main():
         if name in ab:
             ent["algorithmic_bytes"] = ab[name]
             ent["achieved_GBs"] = ab[name] / avg_s / 1e9
+            ent["compact_bytes"] = cb[name]
+            ent["compact_GBs"] = cb[name] / avg_s / 1e9
         kernels[name] = ent
     dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["ms_per_step"])
 
@@ -349,8 +374,12 @@ def main():
     traffic = read_traffic(dom) if (args.n == 4096 and p.rko == 4) else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-            "avg_us": dom_avg_s * 1e6, "launches": dom_cnt}
+            "avg_us": dom_avg_s * 1e6, "launches": dom_cnt,
+            # the bytes the compact layout moves instead, and the measured HBM rate
+            "compact_bytes": cb[dom], "achieved_compact": cb[dom] / dom_avg_s / 1e9,
+            "hbm_measured": None if traffic is None else traffic / dom_avg_s / 1e9}
     step_bytes = sum(ab[k] * kernels[k]["launches_per_step"] for k in ab if k in kernels)
+    step_compact = sum(cb[k] * kernels[k]["launches_per_step"] for k in cb if k in kernels)
     out = {
         "metric": METRIC,
         "value": rate,
@@ -370,7 +399,10 @@ def main():
         "roofline": roof,
         "step_model": {"algorithmic_bytes_per_step": step_bytes,
                        "bytes_per_gp_step": step_bytes / (p.n * p.n),
-                       "hbm_GBs_at_measured_rate": step_bytes * args.steps / el / 1e9},
+                       "GBs_at_measured_rate": step_bytes * args.steps / el / 1e9,
+                       "frac": step_bytes * args.steps / el / 1e9 / HBM_PEAK_GBS,
+                       "compact_bytes_per_step": step_compact,
+                       "compact_bytes_per_gp_step": step_compact / (p.n * p.n)},
         "kernels": kernels,
         "kernels_sample_steps": args.breakdown_steps,
     }

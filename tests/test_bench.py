@@ -29,14 +29,29 @@ def test_committed_counters_match_the_byte_model():
     w = workloads()["euler4096_rk4"]
     n = w.params.n
     m = n // 3  # the 2/3 rectangle: kx, ky in [-N/3, N/3]
-    ab = bench.algorithmic_bytes(n, (m, m, m), 4)
-    for k, v in ab.items():
+    cb = bench.compact_bytes(n, (m, m, m), 4)
+    for k, v in cb.items():
         t = bench.read_traffic(k)
         assert t is not None, k
         assert abs(t / v - 1.0) < 0.02, (k, t, v)
     # one RK4 step: y_start + 3 y_stage + y_final + 4 x passes
-    per_step = ab["y_start"] + 3 * ab["y_stage"] + ab["y_final"] + 4 * ab["x_pass"]
+    per_step = cb["y_start"] + 3 * cb["y_stage"] + cb["y_final"] + 4 * cb["x_pass"]
     assert 250 < per_step / n ** 2 < 265
+
+
+@pytest.mark.parametrize("rko,per_gp", [(4, 468.0), (2, 216.0)])
+def test_fixed_byte_model_per_step(rko, per_gp):
+    """The fixed byte model distributed over the kernel classes adds up to
+    58.5A (RK4) and 27A (RK2) per step, A = 8 N^2 bytes."""
+    sys.path.insert(0, ROOT)
+    import bench
+    n = 1024
+    ab = bench.model_bytes(n, rko)
+    if rko == 4:
+        per_step = ab["y_start"] + 3 * ab["y_stage"] + ab["y_final"] + 4 * ab["x_pass"]
+    else:
+        per_step = ab["y_start"] + ab["y_stage"] + ab["y_final"] + 2 * ab["x_pass"]
+    assert per_step / n ** 2 == pytest.approx(per_gp, rel=1e-12)
 
 
 @pytest.mark.gpu
