@@ -332,8 +332,12 @@ def test_4096_nlt_and_steps_on_each_transform_path(oracle_mod, alpv, truncate):
 def test_sqg1024_workload_with_diagnostics_matches_oracle(oracle_mod):
     """The 1024^2 SQG workload (alpha=1, RK4, dt=2e-4, nu=1e-22 k^8,
     mu=0.05, ring (31,33], seed-1 initial field, spectra and fluxes every 100
-    steps) over its first 200 steps: vorticity within 1e-10, spectra, fluxes
-    and global sums within 1e-9 of the oracle."""
+    steps) over its first 1000 steps: vorticity within 1e-10, spectra, fluxes
+    and global sums within 1e-9 of the oracle.  (Its 2000 steps are past the
+    flow's predictability for FP64 rounding: two runs on the GPU from initial
+    fields 6e-16 apart separate to 1e-10 near step 1350 and 1e-4 at step 2000,
+    growing about 8x per 100 steps late in the run, as the GPU and the oracle
+    do; 1000 steps keep that separation near 1e-12.)"""
     from gturbo_amd import ic, io, workloads
     w = workloads()["sqg1024"]
     p = w.params
@@ -341,15 +345,16 @@ def test_sqg1024_workload_with_diagnostics_matches_oracle(oracle_mod):
     zg = io.field_to_ref(ic.physical_field(p, w.ic))
     zo = zg.copy()
     got = []
-    print("sqg1024, 200 steps", flush=True)
-    seed_g, _ = s.run(zg, w.forcing_seed, 200, diag=lambda *d: got.append(d))
-    seed_o, _, diag = o.run(zo, w.forcing_seed, 200, with_diag=True)
+    nsteps = 1000
+    print(f"sqg1024, {nsteps} steps", flush=True)
+    seed_g, _ = s.run(zg, w.forcing_seed, nsteps, diag=lambda *d: got.append(d))
+    seed_o, _, diag = o.run(zo, w.forcing_seed, nsteps, with_diag=True)
     assert seed_g == seed_o
     err = rel_l2(zg[:, :p.n], zo[:, :p.n])
-    print(f"sqg1024 200-step rel L2 = {err:.3e}")
+    print(f"sqg1024 {nsteps}-step rel L2 = {err:.3e}")
     assert err < 1e-10
     nb = p.n // 2
-    assert len(got) == diag.shape[0] == 2
+    assert len(got) == diag.shape[0] == nsteps // p.imix
     for (ipas, sp, sz, fle, flz, g), row in zip(got, diag):
         osp, osz, ofle, oflz = (row[i * nb:(i + 1) * nb] for i in range(4))
         assert rel_l2(sp, osp) < 1e-9 and rel_l2(sz, osz) < 1e-9
