@@ -329,6 +329,43 @@ def test_4096_nlt_and_steps_on_each_transform_path(oracle_mod, alpv, truncate):
         s.close()
 
 
+@pytest.mark.parametrize("alpv", [2.0, 1.0])
+def test_8192_nlt_and_steps(oracle_mod, alpv):
+    """N = 8192, the reference's compile-time default (src/modules.f90): nlt,
+    one RK4 step and one RK2 step with forcing against the oracle."""
+    n = 8192
+    for rko in (4, 2):
+        p = params_for(n, alpv=alpv, rko=rko, ring=(63.0, 65.0))  # nf within NFORC
+        s, o = _pair(oracle_mod, p)
+        z = random_spectral(n, 41 + rko)
+        zg = z.copy()
+        u, v = s.field(), s.field()
+        scratch = [s.field() for _ in range(4)]
+        _present(s, zg, u, v, *scratch)
+        print("8192 alpha", alpv, "rko", rko, flush=True)
+        if rko == 4:
+            zn, zng = z.copy(), z.copy()
+            _present(s, zng)
+            s.nlt(zng, u, v)
+            s.update_host(zng)
+            o.nlt(zn)
+            assert rel_l2(zng, zn) < 1e-12
+            s.exit_data(zng, copyout=False)
+            s.step4(zg, *scratch)
+            o.step4(z)
+        else:
+            s.step(zg, scratch[0], scratch[2], scratch[3])
+            o.step(z)
+        seed_g = s.forcing(zg, 481929)
+        seed_o = o.forcing(z, 481929)
+        s.update_host(zg)
+        assert seed_g == seed_o
+        err = rel_l2(zg, z)
+        print(f"  step rel L2 = {err:.3e}", flush=True)
+        assert err < 1e-12
+        s.close()
+
+
 def test_sqg1024_workload_with_diagnostics_matches_oracle(oracle_mod):
     """The 1024^2 SQG workload (alpha=1, RK4, dt=2e-4, nu=1e-22 k^8,
     mu=0.05, ring (31,33], seed-1 initial field, spectra and fluxes every 100
