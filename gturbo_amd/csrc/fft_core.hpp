@@ -313,14 +313,14 @@ __device__ __forceinline__ void apply_tw8(double2* t, double2 w1) {
 
 // bar: barrier before the first LDS write (another wave may still read
 // LDS); a uniform run-time flag, so that a loop holds one copy of the code.
-template <int S>
+template <int S, int TS = 1>
 __device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
                                         const double2* __restrict__ tw, bool bar) {
   asm volatile("" : "+v"(j));
-  double2 wn = tw[j];  // pass 1: W^(j r)
+  double2 wn = tw[TS * j];  // pass 1: W^(j r)
   dft<8, S>(a);
   apply_tw8<S>(a, wn);
-  wn = tw[8 * (j & 63)];  // pass 2, in flight across the barrier
+  wn = tw[TS * 8 * (j & 63)];  // pass 2, in flight across the barrier
   if (bar) __syncthreads();
   {
     double2* p = buf + ip_pad(j);
@@ -334,7 +334,7 @@ __device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
     for (int r = 0; r < 8; ++r) a[r] = p[72 * r];
     dft<8, S>(a);
     apply_tw8<S>(a, wn);
-    wn = tw[64 * (j & 7)];
+    wn = tw[TS * 64 * (j & 7)];
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[72 * r] = a[r];
   }
@@ -362,11 +362,11 @@ __device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
 // elements: a barrier must precede the next LDS write.
 // BAR: start with a workgroup barrier (another wave may still read this
 // wave's run of LDS, e.g. after a preceding fft_dit).
-template <int S, bool BAR = false>
+template <int S, bool BAR = false, int TS = 1>
 __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
                                         const double2* __restrict__ tw) {
   asm volatile("" : "+v"(j));
-  double2 wn = tw[64 * (j & 7)];
+  double2 wn = tw[TS * 64 * (j & 7)];
   dft<8, S>(a);  // pass 4, stride 1
   if constexpr (BAR) __syncthreads();
   {
@@ -380,7 +380,7 @@ __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
 #pragma unroll
     for (int r = 0; r < 8; ++r) a[r] = p[9 * r];
     apply_tw8<S>(a, wn);
-    wn = tw[8 * (j & 63)];
+    wn = tw[TS * 8 * (j & 63)];
     dft<8, S>(a);
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[9 * r] = a[r];
@@ -391,7 +391,7 @@ __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
 #pragma unroll
     for (int r = 0; r < 8; ++r) a[r] = p[72 * r];
     apply_tw8<S>(a, wn);
-    wn = tw[j];
+    wn = tw[TS * j];
     dft<8, S>(a);
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[72 * r] = a[r];
@@ -403,6 +403,66 @@ __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
     for (int r = 0; r < 8; ++r) a[r] = p[576 * r];
     apply_tw8<S>(a, wn);
     dft<8, S>(a);
+  }
+}
+
+// 8192-point lines over 1024 threads (thread t holds x[t + 1024 r]): a
+// radix-2 pass (stride 4096) on the thread's own elements splits the line
+// into two 4096-point problems, one exchange hands problem h to threads
+// 512h..512h+511 in natural order, and each half runs the in-place
+// transform above with the 8192-point table read at stride 2.  fft_dif2
+// leaves thread (h, j), register r with X[2 (D(j) + 512 r) + h]; fft_dit2
+// takes that order and returns natural order.  buf: two kIpLine regions.
+template <int S>
+__device__ __forceinline__ void fft_dif2(double2* a, double2* buf, int t,
+                                         const double2* __restrict__ tw, bool bar) {
+  asm volatile("" : "+v"(t));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {  // Y0[n] = x[n] + x[n+4096], Y1[n] = (x[n] - x[n+4096]) W^n
+    const double2 d = csub(a[r], a[r + 4]);
+    a[r] = cadd(a[r], a[r + 4]);
+    const double2 w = tw[t + 1024 * r];
+    a[r + 4] = S < 0 ? cmul(d, w) : cmulc(d, w);
+  }
+  if (bar) __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 8; ++r)  // Y_(r/4)[t + 1024 (r%4)] into region r/4
+    buf[(r >> 2) * kIpLine + ip_pad(t + 1024 * (r & 3))] = a[r];
+  __syncthreads();
+  const int j = t & 511;
+  double2* hb = buf + (t >> 9) * kIpLine;
+  {
+    const double2* p = hb + ip_pad(j);
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[r] = p[576 * r];  // ip_pad(j + 512 r)
+  }
+  // the first pass of fft_dif writes back exactly the positions just read
+  fft_dif<S, 2>(a, hb, j, tw, false);
+}
+
+// Ends after reads of every wave's elements: a barrier must precede the
+// next LDS write.
+template <int S>
+__device__ __forceinline__ void fft_dit2(double2* a, double2* buf, int t,
+                                         const double2* __restrict__ tw) {
+  asm volatile("" : "+v"(t));
+  const int j = t & 511;
+  double2* hb = buf + (t >> 9) * kIpLine;
+  fft_dit<S, false, 2>(a, hb, j, tw);  // natural order within the half
+  {
+    double2* p = hb + ip_pad(j);  // the positions fft_dit's last pass read
+#pragma unroll
+    for (int r = 0; r < 8; ++r) p[576 * r] = a[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {  // x[n] = Y0[n] + W^n Y1[n], x[n+4096] = Y0[n] - W^n Y1[n]
+    const int e = ip_pad(t + 1024 * r);
+    const double2 y0 = buf[e];
+    const double2 w = tw[t + 1024 * r];
+    const double2 y1 = S < 0 ? cmul(buf[kIpLine + e], w) : cmulc(buf[kIpLine + e], w);
+    a[r] = cadd(y0, y1);
+    a[r + 4] = csub(y0, y1);
   }
 }
 

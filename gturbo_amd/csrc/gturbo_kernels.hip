@@ -469,15 +469,18 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   // the inverse ones leave x out of order, which the pointwise products do
   // not mind, and the forward ones restore natural order for split_store.
   constexpr bool IP = LOGN == 12 && NL == 1 && TWPF;
+  constexpr bool IP2 = LOGN == 13 && NL == 1 && TWPF;  // 8192: radix 2, then two 4096s
   // bar: another wave may still read LDS (ignored by the Stockham transforms,
   // which synchronise on every transpose)
   auto inv = [&](double2* v, bool bar) {
     if constexpr (IP) fft_dif<+1>(v, buf, j, tw, bar);
+    else if constexpr (IP2) fft_dif2<+1>(v, buf, j, tw, bar);
     else fft_reg<LOGN, +1, SCHED, TWPF>(v, buf, j, tw);
   };
   auto fwd = [&](double2* v) {
-    if constexpr (IP) {
-      fft_dit<-1>(v, buf, j, tw);
+    if constexpr (IP || IP2) {
+      if constexpr (IP) fft_dit<-1>(v, buf, j, tw);
+      else fft_dit2<-1>(v, buf, j, tw);
       __syncthreads();  // every wave's reads done before split_store writes
     } else {
       fft_reg<LOGN, -1, SCHED, TWPF>(v, buf, j, tw);
@@ -520,7 +523,7 @@ __global__ void __launch_bounds__((1 << LOGN) / kE, 4)
     r_pass_pair(PassArgs a) {
   // Twiddles from memory: these workgroups are short-lived (one row pair),
   // and staging the table in LDS measured slower here.
-  __shared__ double2 lds[LOGN == 12 ? kIpLine : Line<LOGN>::PADDED];
+  __shared__ double2 lds[LOGN == 12 ? kIpLine : LOGN == 13 ? 2 * kIpLine : Line<LOGN>::PADDED];
   pair_body<LOGN, TR, bt_of(LOGN), 1, 0, true>(a, xcd_remap(blockIdx.x, gridDim.x),
                                                threadIdx.x, lds, a.tw);
 }
