@@ -141,3 +141,54 @@ def test_in_place_layout_and_ownership():
         assert all(lo <= p < lo + 576 for p in p2)
         run = ip_pad(64 * (j >> 3))
         assert all(run <= p < run + 72 for p in p3 + p4)
+
+
+def _tw8192(m, s):
+    return np.exp(s * 2j * np.pi * np.asarray(m) / 8192)
+
+
+def _dif2(a, s):
+    """fft_dif2: radix 2 on the thread's elements t + 1024 r, then 4096-point
+    fft_dif on each half of the threads."""
+    y = np.zeros((2, 4096), complex)
+    for t in range(1024):
+        for r in range(4):
+            n = t + 1024 * r
+            y[0][n] = a[t][r] + a[t][r + 4]
+            y[1][n] = (a[t][r] - a[t][r + 4]) * _tw8192(n, s)
+    out = np.zeros((1024, 8), complex)
+    for h in range(2):
+        nat = np.array([[y[h][j + 512 * r] for r in range(8)] for j in range(512)])
+        out[512 * h:512 * h + 512] = _dif(nat, s)
+    return out
+
+
+def _dit2(a, s):
+    y = np.zeros((2, 4096), complex)
+    for h in range(2):
+        nat = _dit(a[512 * h:512 * h + 512], s)
+        for j in range(512):
+            y[h][j + 512 * np.arange(8)] = nat[j]
+    out = np.zeros((1024, 8), complex)
+    for t in range(1024):
+        for r in range(4):
+            n = t + 1024 * r
+            y1 = y[1][n] * _tw8192(n, s)
+            out[t][r], out[t][r + 4] = y[0][n] + y1, y[0][n] - y1
+    return out
+
+
+@pytest.mark.parametrize("s", [+1, -1])
+def test_split_8192_transforms_reproduce_the_dft(s):
+    """fft_dif2 leaves thread (h, j), register r with X[2 (D(j) + 512 r) + h];
+    fft_dit2 takes that order back to natural order."""
+    n = 8192
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.ifft(x) * n if s > 0 else np.fft.fft(x)
+    order = np.array([[2 * (D(t & 511) + 512 * r) + (t >> 9) for r in range(8)]
+                      for t in range(1024)])
+    nat = np.array([[x[t + 1024 * r] for r in range(8)] for t in range(1024)])
+    assert np.abs(_dif2(nat, s) - ref[order]).max() < 1e-9 * np.abs(ref).max()
+    natural = np.arange(n).reshape(8, 1024).T  # [t][r] = t + 1024 r
+    assert np.abs(_dit2(x[order], s) - ref[natural]).max() < 1e-9 * np.abs(ref).max()
