@@ -441,14 +441,15 @@ __device__ __forceinline__ void fft_dif2(double2* a, double2* buf, int t,
 }
 
 // Ends after reads of every wave's elements: a barrier must precede the
-// next LDS write.
-template <int S>
+// next LDS write.  BAR: start with a barrier (another wave may still read
+// this wave's run, e.g. after a preceding fft_dit2).
+template <int S, bool BAR = false>
 __device__ __forceinline__ void fft_dit2(double2* a, double2* buf, int t,
                                          const double2* __restrict__ tw) {
   asm volatile("" : "+v"(t));
   const int j = t & 511;
   double2* hb = buf + (t >> 9) * kIpLine;
-  fft_dit<S, false, 2>(a, hb, j, tw);  // natural order within the half
+  fft_dit<S, BAR, 2>(a, hb, j, tw);  // natural order within the half
   {
     double2* p = hb + ip_pad(j);  // the positions fft_dit's last pass read
 #pragma unroll

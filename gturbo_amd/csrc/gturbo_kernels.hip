@@ -127,15 +127,19 @@ template <int LOGN, int MODE, int AK, int TR, int BT, int SCHED, bool IP = false
 __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
                                        double2* lds, const double2* tw) {
   constexpr int N = 1 << LOGN, TPL = N / kE;
-  constexpr int LINE = IP ? kIpLine : Line<LOGN>::PADDED;
-  static_assert(!IP || (LOGN == 12 && BT == 2), "in-place y pass: 4096-point, two lines");
+  constexpr int LINE = IP ? (LOGN == 13 ? 2 : 1) * kIpLine : Line<LOGN>::PADDED;
+  static_assert(!IP || (LOGN == 12 && BT == 2) || (LOGN == 13 && BT == 1),
+                "in-place y pass: two 4096-point lines or one 8192-point line");
   constexpr bool FWD = MODE != C_START;
   constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
                        MODE == C_RK4_3 || MODE == C_RK2_1;
   int b, j;
-  if constexpr (IP) {
+  if constexpr (IP && BT == 2) {
     b = (tid >> 6) & 1;
     j = ((tid >> 7) << 6) | (tid & 63);
+  } else if constexpr (IP) {
+    b = 0;
+    j = tid;
   } else {
     line_map<TPL, BT>(tid, b, j);
   }
@@ -156,14 +160,16 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
     const double2* srcA = a.y2;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcA[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
-    if constexpr (IP) fft_dit<-1, false>(z, buf, j, tw);
+    if constexpr (IP && LOGN == 12) fft_dit<-1, false>(z, buf, j, tw);
+    else if constexpr (IP) fft_dit2<-1, false>(z, buf, j, tw);
     else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
 #pragma unroll
     for (int r = 0; r < kE; ++r) A[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
     const double2* srcB = a.y2 + (size_t)a.ntiles * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcB[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
-    if constexpr (IP) fft_dit<-1, true>(z, buf, j, tw);
+    if constexpr (IP && LOGN == 12) fft_dit<-1, true>(z, buf, j, tw);
+    else if constexpr (IP) fft_dit2<-1, true>(z, buf, j, tw);
     else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
     int je = j;
     // The epilogue's indices (and so its loads) depend on the transform's
@@ -292,7 +298,8 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
       }
       if constexpr (IP) {
         // a barrier first unless nothing has used LDS since the table staging
-        fft_dif<+1>(z, buf, j, tw, FWD || f > 0);
+        if constexpr (LOGN == 12) fft_dif<+1>(z, buf, j, tw, FWD || f > 0);
+        else fft_dif2<+1>(z, buf, j, tw, FWD || f > 0);
       } else {
         fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
       }
@@ -311,8 +318,8 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
   // Row slots in transform order (in-place transforms) for alpha = 2 on the
   // 2/3 rectangle; every y pass of a run has the same AK and TR, so the
   // buffers' row order is the same in all of them.
-  constexpr bool IP = LOGN == 12 && AK == 1 && TR == 1;
-  __shared__ double2 lds[BT * (IP ? kIpLine : Line<LOGN>::PADDED)];
+  constexpr bool IP = (LOGN == 12 || LOGN == 13) && AK == 1 && TR == 1;
+  __shared__ double2 lds[IP ? 2 * kIpLine : BT * Line<LOGN>::PADDED];
   __shared__ double2 twl[tw_lds_size<LOGN>()];
   const double2* tw = stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
   c_body<LOGN, MODE, AK, TR, BT, 0, IP>(a, blockIdx.x, threadIdx.x, lds, tw);
