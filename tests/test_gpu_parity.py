@@ -213,6 +213,7 @@ def test_diagnostics_match_oracle(oracle_mod, n, alpv):
     (256, 2 * np.pi, 2 * np.pi, 0.5, -1.0, 2.0, 2.0, 3),    # smaller radius, drag mu k^-2
     (256, 2 * np.pi, 2 * np.pi, 0.9, 0.0, 4.0, 0.5, 3),     # past 2/3: full rectangle
     (4096, 2 * np.pi, 2 * np.pi, 0.9, 0.0, 4.0, 2.0, 1),    # the same at 4096
+    (4096, 2 * np.pi, 2 * np.pi, 0.5, 0.0, 4.0, 2.0, 1),    # in-place y path, smaller rect
 ])
 def test_other_parameters_steps_and_diagnostics(oracle_mod, n, xlx, xly, rtrunc, alpmu,
                                                 alpnu, alpv, steps):
