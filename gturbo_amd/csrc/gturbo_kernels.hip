@@ -303,10 +303,17 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
       } else {
         fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
       }
-      double2* dst = a.y1 + (size_t)f * a.ntiles * N * BT;
-      const unsigned colo = ycol_off(c, BT);
+      if constexpr (LOGN == 12) {  // row-pair blocks (y1_off)
+        const int cols = y1_cols(rc.kc());
+        double2* dst = a.y1 + (size_t)f * N * cols;
 #pragma unroll
-      for (int r = 0; r < kE; ++r) dst[yrow_off(j + TPL * r, a.ntiles, BT) + colo] = z[r];
+        for (int r = 0; r < kE; ++r) dst[y1_off(j + TPL * r, c, cols)] = z[r];
+      } else {
+        double2* dst = a.y1 + (size_t)f * a.ntiles * N * BT;
+        const unsigned colo = ycol_off(c, BT);
+#pragma unroll
+        for (int r = 0; r < kE; ++r) dst[yrow_off(j + TPL * r, a.ntiles, BT) + colo] = z[r];
+      }
     }
   }
 }
@@ -341,7 +348,8 @@ __device__ __forceinline__ unsigned koff(int k) {
 // transform).  The inverse transform of Z is f + i g.  Built in two
 // halves: the loads (zeros outside the kept half-spectrum), all issued
 // before any is used so that they are in flight together, and the packing.
-template <int LOGN, int TR, int BT>
+// Y1: the input is in row-pair blocks (y1_off) rather than tiles.
+template <int LOGN, int TR, int BT, bool Y1 = false>
 __device__ __forceinline__ void load_two(double2* x0, double2* x1, int j,
                                          const double2* __restrict__ y,
                                          unsigned off_f, unsigned off_g, int kxm) {
@@ -350,10 +358,11 @@ __device__ __forceinline__ void load_two(double2* x0, double2* x1, int j,
   for (int r = 0; r < kE; ++r) {
     const int p = j + TPL * r;
     const int k = p <= N / 2 ? p : N - p;
+    const unsigned ko = Y1 ? y1_off(0, k, 0) : koff<LOGN, BT>(k);
     x0[r] = x1[r] = make_double2(0.0, 0.0);
     if (live_r<TR>(r) && k <= kxm) {
-      x0[r] = y[off_f + koff<LOGN, BT>(k)];
-      x1[r] = y[off_g + koff<LOGN, BT>(k)];
+      x0[r] = y[off_f + ko];
+      x1[r] = y[off_g + ko];
     }
   }
 }
@@ -369,12 +378,12 @@ __device__ __forceinline__ void pack_two(double2* z, int j, const double2* x0,
     else z[r] = make_double2(x0[r].x + x1[r].y, x1[r].x - x0[r].y);
   }
 }
-template <int LOGN, int TR, int BT>
+template <int LOGN, int TR, int BT, bool Y1 = false>
 __device__ __forceinline__ void build_two(double2* z, int j,
                                           const double2* __restrict__ y,
                                           unsigned off_f, unsigned off_g, int kxm) {
   double2 x0[kE], x1[kE];
-  load_two<LOGN, TR, BT>(x0, x1, j, y, off_f, off_g, kxm);
+  load_two<LOGN, TR, BT, Y1>(x0, x1, j, y, off_f, off_g, kxm);
   pack_two<LOGN>(z, j, x0, x1);
 }
 // Element (N - k) mod N, k = j + TPL r (r a constant), of a line in LDS;
@@ -468,9 +477,14 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   double2* buf = lds + pl * LINE;
   const int y0 = 2 * (g * NL + pl);
   const int kxm = a.rc.kx;
-  const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field
-  const unsigned r0 = yrow_off(y0, a.ntiles, BT), r1 = r0 + BT;  // rows y0, y0+1
-  const unsigned fq = 0, fu = fstride, fv = 2 * fstride;
+  const unsigned fstride = (unsigned)a.ntiles * N * BT;  // one field of y2
+  const unsigned r0 = yrow_off(y0, a.ntiles, BT), r1 = r0 + BT;  // rows y0, y0+1 of y2
+  // y1: row-pair blocks at N = 4096 (the y pass's layout there), else tiles
+  constexpr bool Y1 = LOGN == 12;
+  const unsigned cols = Y1 ? (unsigned)y1_cols(a.rc.kc()) : 0u;
+  const unsigned fs1 = Y1 ? N * cols : fstride;
+  const unsigned i0 = Y1 ? y1_off(y0, 0, cols) : r0, i1 = Y1 ? y1_off(y0 + 1, 0, cols) : r1;
+  const unsigned fq = 0, fu = fs1, fv = 2 * fs1;
 
   // 4096-point lines of a one-pair workgroup use the in-place transforms:
   // the inverse ones leave x out of order, which the pointwise products do
@@ -494,17 +508,17 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
     }
   };
   double2 z[kE], qq[kE];
-  build_two<LOGN, TR, BT>(z, j, a.y1, fq + r0, fq + r1, kxm);  // Q0 + i Q1
+  build_two<LOGN, TR, BT, Y1>(z, j, a.y1, fq + i0, fq + i1, kxm);  // Q0 + i Q1
   inv(z, false);  // no LDS reader yet
 #pragma unroll
   for (int r = 0; r < kE; ++r) qq[r] = z[r];                   // q0 + i q1
-  build_two<LOGN, TR, BT>(z, j, a.y1, fv + r0, fv + r1, kxm);  // V0 + i V1
+  build_two<LOGN, TR, BT, Y1>(z, j, a.y1, fv + i0, fv + i1, kxm);  // V0 + i V1
   inv(z, true);
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
   fwd(z);                                                      // (q v)0 + i (q v)1
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, fstride + r0, fstride + r1, kxm);
-  build_two<LOGN, TR, BT>(z, j, a.y1, fu + r0, fu + r1, kxm);  // U0 + i U1
+  build_two<LOGN, TR, BT, Y1>(z, j, a.y1, fu + i0, fu + i1, kxm);  // U0 + i U1
   inv(z, false);  // split_store ends with a barrier
 #pragma unroll
   for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);

@@ -215,7 +215,9 @@ int rect_tables(gt_ctx* c, const Rect& r, RectTables** out) {
 int ensure_ybuf(gt_ctx* c, const Rect& r) {
   const int bt = gt::bt_of(c->logn);
   const size_t ntiles = (size_t)(r.kc() + bt - 1) / bt;
-  const size_t need = ntiles * bt * (size_t)c->n;
+  // columns of a field: the tiles' (y1, y2), or the y1 blocks' at N = 4096
+  const size_t cols = std::max(ntiles * bt, (size_t)gt::y1_cols(r.kc()));
+  const size_t need = cols * (size_t)c->n;
   if (need <= c->ycap) return GT_OK;
   if (c->y1) hipFree(c->y1);
   if (c->y2) hipFree(c->y2);

@@ -7,7 +7,9 @@
   `fft_dit`): a step-by-step simulation of their passes, with the same LDS
   positions, twiddles and thread/register ownership, reproduces the DFT,
   DIF leaving X[D(j) + 512 r] in thread j, register r, and DIT taking that
-  order back to natural order; passes 2-4 touch only the wave's own run.
+  order back to natural order; passes 2-4 touch only the wave's own run;
+* the row-pair block layout of the inverse-y-pass output at N = 4096
+  (`y1_off`).
 """
 import numpy as np
 import pytest
@@ -192,3 +194,42 @@ def test_split_8192_transforms_reproduce_the_dft(s):
     assert np.abs(_dif2(nat, s) - ref[order]).max() < 1e-9 * np.abs(ref).max()
     natural = np.arange(n).reshape(8, 1024).T  # [t][r] = t + 1024 r
     assert np.abs(_dit2(x[order], s) - ref[natural]).max() < 1e-9 * np.abs(ref).max()
+
+
+def y1_cols(kc):
+    return (kc + 3) // 4 * 4
+
+
+def y1_off(y, c, cols):
+    """Element (row y, column c) of one y1 field at N = 4096 (blocks of 2
+    rows x 4 columns, the two rows of a column adjacent)."""
+    return (y >> 1) * (2 * cols) + (c // 4) * 8 + (c % 4) * 2 + (y & 1)
+
+
+@pytest.mark.parametrize("kc", [1366, 1365, 1024, 683])
+def test_row_pair_blocks_of_the_y1_fields(kc):
+    """The y1 layout at 4096: a bijection onto one field; a row pair's four
+    consecutive columns fill one aligned 128-byte line (8 elements), so the
+    x pass reads whole lines; the y pass's lanes 2i, 2i+1 (rows s, s+1 of a
+    column) write 32 contiguous bytes; the two columns of a tile fill half
+    of each line they touch; and every column a tile writes fits the
+    allocation (tiles of two columns, so up to kc + 1 columns)."""
+    n, cols = 4096, y1_cols(kc)
+    y, c = np.meshgrid(np.arange(n), np.arange(cols), indexing="ij")
+    off = y1_off(y, c, cols)
+    assert np.array_equal(np.sort(off.ravel()), np.arange(n * cols))
+    for y0 in (0, 2, 2046, 4094):
+        for c0 in (0, 4, cols - 4):
+            line = {y1_off(y0 + dy, c0 + dc, cols) for dy in (0, 1) for dc in range(4)}
+            lo = min(line)
+            assert lo % 8 == 0 and line == set(range(lo, lo + 8))
+    s = np.arange(0, n, 2)
+    for col in (0, 1, 5, kc - 1):
+        assert np.all(y1_off(s + 1, col, cols) == y1_off(s, col, cols) + 1)
+    for t in (0, 1, (kc - 1) // 2):
+        tile = {y1_off(y0, 2 * t + b, cols) // 8 * 8 for y0 in (0, 1) for b in (0, 1)}
+        assert len(tile) == 1
+        half = {y1_off(y0, 2 * t + b, cols) for y0 in (0, 1) for b in (0, 1)}
+        assert len(half) == 4
+    ntiles = (kc + 1) // 2
+    assert 2 * ntiles <= cols  # the last tile's second column is inside the field
