@@ -311,16 +311,45 @@ __device__ __forceinline__ void apply_tw8(double2* t, double2 w1) {
   ap(t[7], cmul(w3, w4));
 }
 
+// Twiddles of the three twiddled passes of a 4096-point in-place transform
+// in thread j: W^j (stride 512), W^(8 (j % 64)) (stride 64), W^(64 (j % 8))
+// (stride 8), from the table read at stride TS.  The same for every
+// transform a thread runs, so a kernel running several may load them once.
+struct IpTw {
+  double2 w1, w2, w3;
+};
+template <int TS = 1>
+__device__ __forceinline__ IpTw ip_tw(const double2* __restrict__ tw, int j) {
+  return {tw[TS * j], tw[TS * 8 * (j & 63)], tw[TS * 64 * (j & 7)]};
+}
+// Twiddle sources for fft_dif / fft_dit: operator()(P) gives the twiddle of
+// pass P (1-3), called where the transform first needs it in flight.
+template <int TS>
+struct TwFromTable {
+  const double2* __restrict__ tw;
+  int j;
+  template <int P>
+  __device__ __forceinline__ double2 at() const {
+    return P == 1 ? tw[TS * j] : P == 2 ? tw[TS * 8 * (j & 63)] : tw[TS * 64 * (j & 7)];
+  }
+};
+struct TwHeld {
+  const IpTw& w;
+  template <int P>
+  __device__ __forceinline__ double2 at() const {
+    return P == 1 ? w.w1 : P == 2 ? w.w2 : w.w3;
+  }
+};
+
 // bar: barrier before the first LDS write (another wave may still read
 // LDS); a uniform run-time flag, so that a loop holds one copy of the code.
-template <int S, int TS = 1>
-__device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
-                                        const double2* __restrict__ tw, bool bar) {
-  asm volatile("" : "+v"(j));
-  double2 wn = tw[TS * j];  // pass 1: W^(j r)
+template <int S, class TW>
+__device__ __forceinline__ void fft_dif_t(double2* a, double2* buf, int j, const TW& T,
+                                          bool bar) {
+  double2 wn = T.template at<1>();  // pass 1: W^(j r)
   dft<8, S>(a);
   apply_tw8<S>(a, wn);
-  wn = tw[TS * 8 * (j & 63)];  // pass 2, in flight across the barrier
+  wn = T.template at<2>();  // pass 2, in flight across the barrier
   if (bar) __syncthreads();
   {
     double2* p = buf + ip_pad(j);
@@ -334,7 +363,7 @@ __device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
     for (int r = 0; r < 8; ++r) a[r] = p[72 * r];
     dft<8, S>(a);
     apply_tw8<S>(a, wn);
-    wn = tw[TS * 64 * (j & 7)];
+    wn = T.template at<3>();
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[72 * r] = a[r];
   }
@@ -356,17 +385,35 @@ __device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
     dft<8, S>(a);
   }
 }
+template <int S, int TS = 1>
+__device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j,
+                                        const double2* __restrict__ tw, bool bar) {
+  asm volatile("" : "+v"(j));
+  fft_dif_t<S>(a, buf, j, TwFromTable<TS>{tw, j}, bar);
+}
+// Held twiddles enter each transform opaque, so that the compiler keeps
+// forming the powers inside it rather than holding them across transforms.
+__device__ __forceinline__ IpTw opaque(IpTw w) {
+  asm volatile("" : "+v"(w.w1.x), "+v"(w.w1.y), "+v"(w.w2.x), "+v"(w.w2.y), "+v"(w.w3.x),
+               "+v"(w.w3.y));
+  return w;
+}
+template <int S>
+__device__ __forceinline__ void fft_dif(double2* a, double2* buf, int j, const IpTw& w,
+                                        bool bar) {
+  asm volatile("" : "+v"(j));
+  const IpTw lw = opaque(w);
+  fft_dif_t<S>(a, buf, j, TwHeld{lw}, bar);
+}
 
 // Input as fft_dif leaves it; the thread's own LDS run is free to write
 // (its last reader was this wave).  Ends after reads of every wave's
 // elements: a barrier must precede the next LDS write.
 // BAR: start with a workgroup barrier (another wave may still read this
 // wave's run of LDS, e.g. after a preceding fft_dit).
-template <int S, bool BAR = false, int TS = 1>
-__device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
-                                        const double2* __restrict__ tw) {
-  asm volatile("" : "+v"(j));
-  double2 wn = tw[TS * 64 * (j & 7)];
+template <int S, bool BAR, class TW>
+__device__ __forceinline__ void fft_dit_t(double2* a, double2* buf, int j, const TW& T) {
+  double2 wn = T.template at<3>();
   dft<8, S>(a);  // pass 4, stride 1
   if constexpr (BAR) __syncthreads();
   {
@@ -380,7 +427,7 @@ __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
 #pragma unroll
     for (int r = 0; r < 8; ++r) a[r] = p[9 * r];
     apply_tw8<S>(a, wn);
-    wn = tw[TS * 8 * (j & 63)];
+    wn = T.template at<2>();
     dft<8, S>(a);
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[9 * r] = a[r];
@@ -391,7 +438,7 @@ __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
 #pragma unroll
     for (int r = 0; r < 8; ++r) a[r] = p[72 * r];
     apply_tw8<S>(a, wn);
-    wn = tw[TS * j];
+    wn = T.template at<1>();
     dft<8, S>(a);
 #pragma unroll
     for (int r = 0; r < 8; ++r) p[72 * r] = a[r];
@@ -404,6 +451,18 @@ __device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
     apply_tw8<S>(a, wn);
     dft<8, S>(a);
   }
+}
+template <int S, bool BAR = false, int TS = 1>
+__device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j,
+                                        const double2* __restrict__ tw) {
+  asm volatile("" : "+v"(j));
+  fft_dit_t<S, BAR>(a, buf, j, TwFromTable<TS>{tw, j});
+}
+template <int S, bool BAR = false>
+__device__ __forceinline__ void fft_dit(double2* a, double2* buf, int j, const IpTw& w) {
+  asm volatile("" : "+v"(j));
+  const IpTw lw = opaque(w);
+  fft_dit_t<S, BAR>(a, buf, j, TwHeld{lw});
 }
 
 // 8192-point lines over 1024 threads (thread t holds x[t + 1024 r]): a

@@ -493,14 +493,17 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   constexpr bool IP2 = LOGN == 13 && NL == 1 && TWPF;  // 8192: radix 2, then two 4096s
   // bar: another wave may still read LDS (ignored by the Stockham transforms,
   // which synchronise on every transpose)
+  // 4096: the three twiddles every transform of the thread uses, loaded once.
+  IpTw w{};
+  if constexpr (IP) w = ip_tw(tw, j);
   auto inv = [&](double2* v, bool bar) {
-    if constexpr (IP) fft_dif<+1>(v, buf, j, tw, bar);
+    if constexpr (IP) fft_dif<+1>(v, buf, j, w, bar);
     else if constexpr (IP2) fft_dif2<+1>(v, buf, j, tw, bar);
     else fft_reg<LOGN, +1, SCHED, TWPF>(v, buf, j, tw);
   };
   auto fwd = [&](double2* v) {
     if constexpr (IP || IP2) {
-      if constexpr (IP) fft_dit<-1>(v, buf, j, tw);
+      if constexpr (IP) fft_dit<-1>(v, buf, j, w);
       else fft_dit2<-1>(v, buf, j, tw);
       __syncthreads();  // every wave's reads done before split_store writes
     } else {
