@@ -153,6 +153,9 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
   const unsigned cbase = (unsigned)c * (unsigned)a.kyp;
   double2 s[kE];
   double2 z[kE];
+  // 4096-point in-place transforms: their three twiddles, loaded once.
+  IpTw ytw{};
+  if constexpr (IP && LOGN == 12) ytw = ip_tw(tw, j);
 
   if constexpr (FWD) {
     double2 A[kE];
@@ -160,7 +163,7 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
     const double2* srcA = a.y2;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcA[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
-    if constexpr (IP && LOGN == 12) fft_dit<-1, false>(z, buf, j, tw);
+    if constexpr (IP && LOGN == 12) fft_dit<-1, false>(z, buf, j, ytw);
     else if constexpr (IP) fft_dit2<-1, false>(z, buf, j, tw);
     else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
 #pragma unroll
@@ -168,7 +171,7 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
     const double2* srcB = a.y2 + (size_t)a.ntiles * N * BT;
 #pragma unroll
     for (int r = 0; r < kE; ++r) z[r] = srcB[yrow_off(j + TPL * r, a.ntiles, BT) + colo];
-    if constexpr (IP && LOGN == 12) fft_dit<-1, true>(z, buf, j, tw);
+    if constexpr (IP && LOGN == 12) fft_dit<-1, true>(z, buf, j, ytw);
     else if constexpr (IP) fft_dit2<-1, true>(z, buf, j, tw);
     else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
     int je = j;
@@ -298,7 +301,7 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
       }
       if constexpr (IP) {
         // a barrier first unless nothing has used LDS since the table staging
-        if constexpr (LOGN == 12) fft_dif<+1>(z, buf, j, tw, FWD || f > 0);
+        if constexpr (LOGN == 12) fft_dif<+1>(z, buf, j, ytw, FWD || f > 0);
         else fft_dif2<+1>(z, buf, j, tw, FWD || f > 0);
       } else {
         fft_reg<LOGN, +1, SCHED>(z, buf, j, tw);
