@@ -329,9 +329,12 @@ __global__ void __launch_bounds__(bt_of(LOGN) * (1 << LOGN) / kE, AK == 1 ? 4 : 
   // 2/3 rectangle; every y pass of a run has the same AK and TR, so the
   // buffers' row order is the same in all of them.
   constexpr bool IP = (LOGN == 12 || LOGN == 13) && AK == 1 && TR == 1;
+  // 4096-point in-place transforms hold their three twiddles in registers
+  // (read once from memory): no table in LDS.
+  constexpr bool HELD = IP && LOGN == 12;
   __shared__ double2 lds[IP ? 2 * kIpLine : BT * Line<LOGN>::PADDED];
-  __shared__ double2 twl[tw_lds_size<LOGN>()];
-  const double2* tw = stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
+  __shared__ double2 twl[HELD ? 1 : tw_lds_size<LOGN>()];
+  const double2* tw = HELD ? a.tw : stage_tw<LOGN>(twl, a.tw, threadIdx.x, blockDim.x);
   c_body<LOGN, MODE, AK, TR, BT, 0, IP>(a, blockIdx.x, threadIdx.x, lds, tw);
 }
 
