@@ -497,11 +497,11 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   // not mind, and the forward ones restore natural order for split_store.
   constexpr bool IP = LOGN == 12 && NL == 1 && TWPF;
   constexpr bool IP2 = LOGN == 13 && NL == 1 && TWPF;  // 8192: radix 2, then two 4096s
-  // bar: another wave may still read LDS (ignored by the Stockham transforms,
-  // which synchronise on every transpose)
   // 4096: the three twiddles every transform of the thread uses, loaded once.
   IpTw w{};
   if constexpr (IP) w = ip_tw(tw, j);
+  // bar: another wave may still read LDS (ignored by the Stockham transforms,
+  // which synchronise on every transpose)
   auto inv = [&](double2* v, bool bar) {
     if constexpr (IP) fft_dif<+1>(v, buf, j, w, bar);
     else if constexpr (IP2) fft_dif2<+1>(v, buf, j, tw, bar);
@@ -535,6 +535,8 @@ __device__ __forceinline__ void pair_body(const PassArgs& a, int g, int tid,
   split_store<LOGN, TR, BT>(z, buf, j, a.y2, r0, r1, kxm);
 }
 
+// Not launched for N >= 2048 (r_pass_pair there; at 4096 the y passes
+// write y1 in row-pair blocks, which only pair_body reads).
 template <int LOGN, int TR>
 __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
     r_pass(PassArgs a) {
