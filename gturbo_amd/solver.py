@@ -35,6 +35,32 @@ def _addr(a):
     raise TypeError(f"cannot take the address of {type(a)}")
 
 
+def _field_addr(a, n):
+    """Address of a field array after checking that its memory really is the
+    row-major (N, N+2) float64 array the library reads and writes: a
+    Fortran-ordered array, a strided view or another dtype with enough bytes
+    would otherwise be silently misread.  Raw integer addresses are taken as
+    they are."""
+    if isinstance(a, int):
+        return a
+    size = n * (n + 2)
+    if isinstance(a, np.ndarray):
+        if a.dtype != np.float64:
+            raise TypeError(f"field arrays must be float64, got {a.dtype}")
+        if not a.flags.c_contiguous:
+            raise TypeError("field arrays must be C-contiguous (row-major (N, N+2))")
+        if a.size != size:
+            raise TypeError(f"field arrays hold N*(N+2) = {size} doubles, got shape {a.shape}")
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        import torch
+        if a.dtype != torch.float64 or not a.is_contiguous() or a.numel() != size:
+            raise TypeError(f"field tensors must be contiguous float64 with N*(N+2) = {size} "
+                            f"elements, got {a.dtype} {tuple(a.shape)}")
+        return a.data_ptr()
+    raise TypeError(f"cannot take the address of {type(a)}")
+
+
 class Solver:
     def __init__(self, params: Params):
         self.L = _lib.load()
@@ -105,84 +131,88 @@ class Solver:
         return p.value
 
     # ---- data clauses -------------------------------------------------------
+    def _fa(self, a):
+        return _field_addr(a, self.n)
+
     def enter_data(self, a, copyin=True):
-        check(self.L.gt_enter_data(self.h, _addr(a), int(a.nbytes), 1 if copyin else 0))
-        self._keep[_addr(a)] = a
+        check(self.L.gt_enter_data(self.h, self._fa(a), int(a.nbytes), 1 if copyin else 0))
+        self._keep[self._fa(a)] = a
 
     def exit_data(self, a, copyout=True):
-        check(self.L.gt_exit_data(self.h, _addr(a), 1 if copyout else 0))
-        self._keep.pop(_addr(a), None)
+        check(self.L.gt_exit_data(self.h, self._fa(a), 1 if copyout else 0))
+        self._keep.pop(self._fa(a), None)
 
     def update_host(self, a):
-        check(self.L.gt_update_host(self.h, _addr(a)))
+        check(self.L.gt_update_host(self.h, self._fa(a)))
 
     def update_device(self, a):
-        check(self.L.gt_update_device(self.h, _addr(a)))
+        check(self.L.gt_update_device(self.h, self._fa(a)))
 
     def map_data(self, t):
         """Attach a device tensor (torch, float64, N*(N+2) elements) as its own
         device copy; the tensor's address is the key."""
         nbytes = t.numel() * t.element_size()
+        self._fa(t)
         check(self.L.gt_map_data(self.h, t.data_ptr(), t.data_ptr(), int(nbytes)))
         self._keep[t.data_ptr()] = t
 
     def unmap_data(self, t):
-        check(self.L.gt_unmap_data(self.h, _addr(t)))
-        self._keep.pop(_addr(t), None)
+        check(self.L.gt_unmap_data(self.h, self._fa(t)))
+        self._keep.pop(self._fa(t), None)
 
     def sync_ref(self, a):
-        check(self.L.gt_sync_ref(self.h, _addr(a)))
+        check(self.L.gt_sync_ref(self.h, self._fa(a)))
 
     def copy(self, dst, src):
-        check(self.L.gt_copy(self.h, _addr(dst), _addr(src)))
+        check(self.L.gt_copy(self.h, self._fa(dst), self._fa(src)))
 
     def synchronize(self):
         check(self.L.gt_synchronize(self.h))
 
     # ---- subroutines ----------------------------------------------------------
     def fft_dir(self, z, plan=None):
-        check(self.L.gt_fft_dir(self.h, _addr(z), self.plan_dir if plan is None else plan))
+        check(self.L.gt_fft_dir(self.h, self._fa(z), self.plan_dir if plan is None else plan))
 
     def fft_inv(self, z, plan=None):
-        check(self.L.gt_fft_inv(self.h, _addr(z), self.plan_inv if plan is None else plan))
+        check(self.L.gt_fft_inv(self.h, self._fa(z), self.plan_inv if plan is None else plan))
 
     def produ(self, z, u, v):
-        check(self.L.gt_produ(self.h, _addr(z), _addr(u), _addr(v)))
+        check(self.L.gt_produ(self.h, self._fa(z), self._fa(u), self._fa(v)))
 
     def trunc(self, z):
-        check(self.L.gt_trunc(self.h, _addr(z)))
+        check(self.L.gt_trunc(self.h, self._fa(z)))
 
     def nlt(self, z, u, v):
-        check(self.L.gt_nlt(self.h, _addr(z), _addr(u), _addr(v)))
+        check(self.L.gt_nlt(self.h, self._fa(z), self._fa(u), self._fa(v)))
 
     def step(self, z, znl, u, v):
-        check(self.L.gt_step(self.h, _addr(z), _addr(znl), _addr(u), _addr(v)))
+        check(self.L.gt_step(self.h, self._fa(z), self._fa(znl), self._fa(u), self._fa(v)))
 
     def step4(self, z, znl, znew, u, v):
-        check(self.L.gt_step4(self.h, _addr(z), _addr(znl), _addr(znew), _addr(u),
-                              _addr(v)))
+        check(self.L.gt_step4(self.h, self._fa(z), self._fa(znl), self._fa(znew), self._fa(u),
+                              self._fa(v)))
 
     def forcing(self, z, seed):
         s = C.c_int32(int(seed))
-        check(self.L.gt_forcing(self.h, _addr(z), C.byref(s)))
+        check(self.L.gt_forcing(self.h, self._fa(z), C.byref(s)))
         return s.value
 
     def spectrum(self, z):
         nb = self.n // 2
         sp, sz = np.zeros(nb), np.zeros(nb)
-        check(self.L.gt_spectrum(self.h, _addr(z), sp.ctypes.data, sz.ctypes.data))
+        check(self.L.gt_spectrum(self.h, self._fa(z), sp.ctypes.data, sz.ctypes.data))
         return sp, sz
 
     def fluxes(self, z, znl):
         nb = self.n // 2
         fle, flz = np.zeros(nb), np.zeros(nb)
-        check(self.L.gt_fluxes(self.h, _addr(z), _addr(znl), fle.ctypes.data,
+        check(self.L.gt_fluxes(self.h, self._fa(z), self._fa(znl), fle.ctypes.data,
                                flz.ctypes.data))
         return fle, flz
 
     def glob(self, z):
         out = np.zeros(6)
-        check(self.L.gt_glob(self.h, _addr(z), out.ctypes.data))
+        check(self.L.gt_glob(self.h, self._fa(z), out.ctypes.data))
         return out
 
     def shell_index(self):

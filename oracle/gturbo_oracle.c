@@ -45,6 +45,7 @@ struct or_ctx {
   int npas, iout, imix, split, rko;
   /* commk (src/modules.f90:55-65) */
   double *emk, *emkx, *emky, *fkxs, *fkx, *fkys, *fky;
+  double *ka; /* k2**(alpv/2) of src/nlt.f90:26-27, taken once per mode */
   int32_t *jc; /* 1-based, as jc(j) */
   double rtrunc, fkxsmax, fkysmax;
   int truncate;
@@ -65,38 +66,65 @@ void or_set_threads(int nthreads) {
 }
 
 /* ------------------------------------------------------------------ */
-/* 1D complex FFT (iterative radix-2), the oracle's own transform.      */
+/* 1D complex FFTs (iterative radix-2), the oracle's own transform.     */
 /* ------------------------------------------------------------------ */
 
-static void fft1(const or_ctx *c, double *a, int sign) {
+/* NB independent length-n transforms at once, real and imaginary parts
+ * apart: element j of transform b is re[j * NB + b] + i im[j * NB + b],
+ * with im = re + n * NB.  Each transform sees exactly the operations of a
+ * lone radix-2 transform (bit reversal, then butterflies with
+ * e^{sign 2 pi i k/len}); the NB-wide inner loops only let the compiler
+ * vectorise across transforms, so the result of one transform does not
+ * depend on NB or on which other transforms share the block. */
+#define FFT_NB 4
+typedef double v4d __attribute__((vector_size(4 * sizeof(double))));
+static inline v4d ld4(const double *p) {
+  v4d v;
+  memcpy(&v, p, sizeof v);
+  return v;
+}
+static inline void st4(double *p, v4d v) { memcpy(p, &v, sizeof v); }
+
+static void fft4(const or_ctx *c, double *re, int sign) {
   const int n = c->n;
+  double *im = re + (size_t)n * FFT_NB;
   for (int i = 0; i < n; i++) {
     int r = c->rev[i];
     if (r > i) {
-      double t0 = a[2 * i], t1 = a[2 * i + 1];
-      a[2 * i] = a[2 * r];
-      a[2 * i + 1] = a[2 * r + 1];
-      a[2 * r] = t0;
-      a[2 * r + 1] = t1;
+      v4d t = ld4(re + i * FFT_NB);
+      st4(re + i * FFT_NB, ld4(re + r * FFT_NB));
+      st4(re + r * FFT_NB, t);
+      t = ld4(im + i * FFT_NB);
+      st4(im + i * FFT_NB, ld4(im + r * FFT_NB));
+      st4(im + r * FFT_NB, t);
     }
   }
   for (int len = 2; len <= n; len <<= 1) {
     int half = len >> 1, step = n / len;
     for (int s = 0; s < n; s += len) {
       for (int k = 0; k < half; k++) {
-        double wr = c->tw[2 * (k * step)];
-        double wi = sign * c->tw[2 * (k * step) + 1];
-        double *p = a + 2 * (s + k), *q = a + 2 * (s + k + half);
-        double xr = q[0] * wr - q[1] * wi;
-        double xi = q[0] * wi + q[1] * wr;
-        q[0] = p[0] - xr;
-        q[1] = p[1] - xi;
-        p[0] = p[0] + xr;
-        p[1] = p[1] + xi;
+        const double wr1 = c->tw[2 * (k * step)];
+        const double wi1 = sign * c->tw[2 * (k * step) + 1];
+        const v4d wr = {wr1, wr1, wr1, wr1}, wi = {wi1, wi1, wi1, wi1};
+        double *pr = re + (size_t)(s + k) * FFT_NB, *pi = im + (size_t)(s + k) * FFT_NB;
+        double *qr = re + (size_t)(s + k + half) * FFT_NB;
+        double *qi = im + (size_t)(s + k + half) * FFT_NB;
+        const v4d ar = ld4(pr), ai = ld4(pi), br = ld4(qr), bi = ld4(qi);
+        /* each lane: the radix-2 butterfly of one transform */
+        const v4d xr = br * wr - bi * wi;
+        const v4d xi = br * wi + bi * wr;
+        st4(qr, ar - xr);
+        st4(qi, ai - xi);
+        st4(pr, ar + xr);
+        st4(pi, ai + xi);
       }
     }
   }
 }
+
+/* Block element access: line b, position j. */
+#define BRE(a, j, b) (a)[(size_t)(j) * FFT_NB + (b)]
+#define BIM(a, j, b) (a)[(size_t)c->n * FFT_NB + (size_t)(j) * FFT_NB + (b)]
 
 static int setup_fft(or_ctx *c) {
   const int n = c->n;
@@ -156,6 +184,13 @@ static void defk(or_ctx *c) {
     double scra = c->nu * pow(fk2, c->alpnu);
     c->emkx[i] = exp(-0.5 * c->dt * scra);
   }
+  /* nlt.f90:26-27: ka = k2**(alpv/2) depends on the mode only; the same
+   * pow call, made once here instead of once per nlt. */
+  const double e = c->alpv / 2.0;
+  c->ka[0] = 1.0; /* the (1,1) mode is skipped (nlt.f90:25) */
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < nh; i++)
+      if (i || j) c->ka[i + (size_t)nh * j] = pow(c->fkxs[i] + c->fkys[j], e);
   c->fkxsmax = c->fkxs[nh - 1]; /* defk.f90:71-73 */
   c->fkysmax = c->fkys[n / 2];
   c->rtrunc = c->rtrunc * c->rtrunc;
@@ -240,6 +275,7 @@ or_ctx *or_create(int n, const double *p) {
   c->dt6 = c->dt / 6.0;
   const int nh = c->nh;
   c->emk = (double *)malloc(sizeof(double) * (size_t)nh * n);
+  c->ka = (double *)malloc(sizeof(double) * (size_t)nh * n);
   c->emkx = (double *)malloc(sizeof(double) * nh);
   c->emky = (double *)malloc(sizeof(double) * n);
   c->fkx = (double *)malloc(sizeof(double) * nh);
@@ -258,6 +294,7 @@ or_ctx *or_create(int n, const double *p) {
 void or_destroy(or_ctx *c) {
   if (!c) return;
   free(c->emk);
+  free(c->ka);
   free(c->emkx);
   free(c->emky);
   free(c->fkx);
@@ -312,28 +349,42 @@ void or_fft_dir(const or_ctx *c, double *z) {
   const size_t ld = (size_t)n + 2;
 #pragma omp parallel
   {
-    double *a = (double *)malloc(sizeof(double) * 2 * n);
+    double *a = (double *)malloc(sizeof(double) * 2 * n * FFT_NB);
 #pragma omp for schedule(static)
-    for (int j = 0; j < n; j++) {
-      double *row = z + ld * j;
-      for (int x = 0; x < n; x++) {
-        a[2 * x] = row[x];
-        a[2 * x + 1] = 0.0;
+    for (int j0 = 0; j0 < n; j0 += FFT_NB) { /* rows, FFT_NB at a time */
+      const int nb = n - j0 < FFT_NB ? n - j0 : FFT_NB;
+      if (nb < FFT_NB) memset(a, 0, sizeof(double) * 2 * n * FFT_NB);
+      for (int b = 0; b < nb; b++) {
+        const double *row = z + ld * (j0 + b);
+        for (int x = 0; x < n; x++) {
+          BRE(a, x, b) = row[x];
+          BIM(a, x, b) = 0.0;
+        }
       }
-      fft1(c, a, 1);
-      memcpy(row, a, sizeof(double) * 2 * nh);
+      fft4(c, a, 1);
+      for (int b = 0; b < nb; b++) {
+        double *row = z + ld * (j0 + b);
+        for (int k = 0; k < nh; k++) {
+          row[2 * k] = BRE(a, k, b);
+          row[2 * k + 1] = BIM(a, k, b);
+        }
+      }
     }
 #pragma omp for schedule(static)
-    for (int i = 0; i < nh; i++) {
-      for (int j = 0; j < n; j++) {
-        a[2 * j] = z[2 * (i + (size_t)nh * j)];
-        a[2 * j + 1] = z[2 * (i + (size_t)nh * j) + 1];
-      }
-      fft1(c, a, 1);
-      for (int j = 0; j < n; j++) {
-        z[2 * (i + (size_t)nh * j)] = a[2 * j];
-        z[2 * (i + (size_t)nh * j) + 1] = a[2 * j + 1];
-      }
+    for (int i0 = 0; i0 < nh; i0 += FFT_NB) { /* columns, FFT_NB at a time */
+      const int nb = nh - i0 < FFT_NB ? nh - i0 : FFT_NB;
+      if (nb < FFT_NB) memset(a, 0, sizeof(double) * 2 * n * FFT_NB);
+      for (int j = 0; j < n; j++)
+        for (int b = 0; b < nb; b++) {
+          BRE(a, j, b) = z[2 * (i0 + b + (size_t)nh * j)];
+          BIM(a, j, b) = z[2 * (i0 + b + (size_t)nh * j) + 1];
+        }
+      fft4(c, a, 1);
+      for (int j = 0; j < n; j++)
+        for (int b = 0; b < nb; b++) {
+          z[2 * (i0 + b + (size_t)nh * j)] = BRE(a, j, b);
+          z[2 * (i0 + b + (size_t)nh * j) + 1] = BIM(a, j, b);
+        }
     }
     free(a);
 #pragma omp for schedule(static)
@@ -352,36 +403,48 @@ void or_fft_inv(const or_ctx *c, double *z) {
   const size_t ld = (size_t)n + 2;
 #pragma omp parallel
   {
-    double *a = (double *)malloc(sizeof(double) * 2 * n);
+    double *a = (double *)malloc(sizeof(double) * 2 * n * FFT_NB);
 #pragma omp for schedule(static)
-    for (int i = 0; i < nh; i++) {
-      for (int j = 0; j < n; j++) {
-        a[2 * j] = z[2 * (i + (size_t)nh * j)];
-        a[2 * j + 1] = z[2 * (i + (size_t)nh * j) + 1];
-      }
-      fft1(c, a, -1);
-      for (int j = 0; j < n; j++) {
-        z[2 * (i + (size_t)nh * j)] = a[2 * j];
-        z[2 * (i + (size_t)nh * j) + 1] = a[2 * j + 1];
-      }
+    for (int i0 = 0; i0 < nh; i0 += FFT_NB) { /* columns, FFT_NB at a time */
+      const int nb = nh - i0 < FFT_NB ? nh - i0 : FFT_NB;
+      if (nb < FFT_NB) memset(a, 0, sizeof(double) * 2 * n * FFT_NB);
+      for (int j = 0; j < n; j++)
+        for (int b = 0; b < nb; b++) {
+          BRE(a, j, b) = z[2 * (i0 + b + (size_t)nh * j)];
+          BIM(a, j, b) = z[2 * (i0 + b + (size_t)nh * j) + 1];
+        }
+      fft4(c, a, -1);
+      for (int j = 0; j < n; j++)
+        for (int b = 0; b < nb; b++) {
+          z[2 * (i0 + b + (size_t)nh * j)] = BRE(a, j, b);
+          z[2 * (i0 + b + (size_t)nh * j) + 1] = BIM(a, j, b);
+        }
     }
 #pragma omp for schedule(static)
-    for (int j = 0; j < n; j++) {
-      double *row = z + ld * j;
-      a[0] = row[0];
-      a[1] = 0.0;
-      a[2 * (n / 2)] = row[2 * (n / 2)];
-      a[2 * (n / 2) + 1] = 0.0;
-      for (int k = 1; k < n / 2; k++) {
-        a[2 * k] = row[2 * k];
-        a[2 * k + 1] = row[2 * k + 1];
-        a[2 * (n - k)] = row[2 * k];
-        a[2 * (n - k) + 1] = -row[2 * k + 1];
+    for (int j0 = 0; j0 < n; j0 += FFT_NB) { /* rows, FFT_NB at a time */
+      const int nb = n - j0 < FFT_NB ? n - j0 : FFT_NB;
+      if (nb < FFT_NB) memset(a, 0, sizeof(double) * 2 * n * FFT_NB);
+      for (int b = 0; b < nb; b++) {
+        const double *row = z + ld * (j0 + b);
+        /* Hermitian extension of the half spectrum */
+        BRE(a, 0, b) = row[0];
+        BIM(a, 0, b) = 0.0;
+        BRE(a, (n / 2), b) = row[2 * (n / 2)];
+        BIM(a, (n / 2), b) = 0.0;
+        for (int k = 1; k < n / 2; k++) {
+          BRE(a, k, b) = row[2 * k];
+          BIM(a, k, b) = row[2 * k + 1];
+          BRE(a, (n - k), b) = row[2 * k];
+          BIM(a, (n - k), b) = -row[2 * k + 1];
+        }
       }
-      fft1(c, a, -1);
-      for (int x = 0; x < n; x++) row[x] = a[2 * x];
-      row[n] = 0.0;
-      row[n + 1] = 0.0;
+      fft4(c, a, -1);
+      for (int b = 0; b < nb; b++) {
+        double *row = z + ld * (j0 + b);
+        for (int x = 0; x < n; x++) row[x] = BRE(a, x, b);
+        row[n] = 0.0;
+        row[n + 1] = 0.0;
+      }
     }
     free(a);
   }
@@ -433,13 +496,11 @@ void or_nlt(const or_ctx *c, double *z, double *u, double *v) {
   const size_t ns = ((size_t)n + 2) * n;
   memset(u, 0, sizeof(double) * ns); /* nlt.f90:17-20 */
   memset(v, 0, sizeof(double) * ns);
-  const double e = c->alpv / 2.0;
 #pragma omp parallel for schedule(static)
   for (int j = 0; j < n; j++) /* nlt.f90:22-36 */
     for (int i = 0; i < nh; i++) {
       if (i == 0 && j == 0) continue;
-      double k2 = c->fkxs[i] + c->fkys[j];
-      double ka = pow(k2, e);
+      double ka = c->ka[i + (size_t)nh * j]; /* k2**(alpv/2) */
       double kx = c->fkx[i] / ka;
       double ky = c->fky[j] / ka;
       size_t p = 2 * (i + (size_t)nh * j);
@@ -542,15 +603,13 @@ void or_step4(const or_ctx *c, double *z, double *znl, double *znew,
 static void velocity(const or_ctx *c, const double *z, double *u, double *v) {
   const int n = c->n, nh = c->nh;
   const size_t ns = ((size_t)n + 2) * n;
-  const double e = c->alpv / 2.0;
   memset(u, 0, sizeof(double) * ns);
   memset(v, 0, sizeof(double) * ns);
 #pragma omp parallel for schedule(static)
   for (int j = 0; j < n; j++)
     for (int i = 0; i < nh; i++) {
       if (i == 0 && j == 0) continue;
-      double k2 = c->fkxs[i] + c->fkys[j];
-      double ka = pow(k2, e);
+      double ka = c->ka[i + (size_t)nh * j];
       double kx = c->fkx[i] / ka;
       double ky = c->fky[j] / ka;
       size_t p = 2 * (i + (size_t)nh * j);

@@ -167,3 +167,22 @@ def test_make_runs_lays_out_run_directories(tmp_path, monkeypatch):
         assert abs(e - 0.5) < 1e-4
         fields.append(w)
     assert not np.array_equal(fields[0], fields[1])
+
+
+def test_field_arrays_are_checked_before_their_address_is_taken():
+    """The library reads and writes a field as row-major (N, N+2) float64
+    memory; an array that is not exactly that must be refused, not misread."""
+    from gturbo_amd.solver import _field_addr
+    n = 16
+    good = np.zeros((n, n + 2))
+    assert _field_addr(good, n) == good.ctypes.data
+    assert _field_addr(good.reshape(-1), n) == good.ctypes.data  # same memory, flat
+    assert _field_addr(12345, n) == 12345                          # raw address as is
+    bad = [np.zeros((n, n + 2), np.float32),                       # wrong dtype
+           np.asfortranarray(np.zeros((n, n + 2))),                # column-major
+           np.zeros((n, 2 * (n + 2)))[:, ::2],                     # strided view
+           np.zeros((n + 2, n + 2)),                               # wrong size
+           np.zeros((n, n + 2), np.complex128)]                    # same bytes x 2
+    for a in bad:
+        with pytest.raises(TypeError):
+            _field_addr(a, n)

@@ -333,8 +333,143 @@ def test_split_step_uses_product_factor(O):
 
 
 # --------------------------------------------------------------------------
+# The integrating factor from the physics alone: src/defk.f90:40-50 and the
+# Lawson steps of src/rkstep.f90.  With N(z) = 0 the step must be the exact
+# solution of dq/dt = -D(k) q over dt, D = nu k^(2 alpnu) + mu k^(2 alpmu):
+# q(dt) = exp(-dt D) q(0).  Plane waves with parallel wave vectors have a
+# vanishing Jacobian (u is normal to k, grad q along it), so a state made of
+# them decays mode by mode.  The expected values below are computed from the
+# params, never from the oracle's tables.
+# --------------------------------------------------------------------------
+
+PARALLEL_MODES = [(1, 2, 0.7 - 0.2j), (2, 4, -0.3 + 0.5j), (3, 6, 0.25 + 0.1j),
+                  (5, 10, 0.05 - 0.02j)]  # (kx, ky, amplitude): k parallel to (1, 2)
+
+
+def _parallel_state(n, xlx=2 * np.pi, xly=2 * np.pi):
+    z = np.zeros((n, n + 2))
+    zc = as_complex(z)
+    for kx, ky, a in PARALLEL_MODES:
+        zc[ky % n, kx] = a
+    return z
+
+
+@pytest.mark.parametrize("rko,alpnu,alpmu,nu,mu", [
+    (4, 4.0, 0.0, 1e-5, 1.0),    # hyperviscosity k^8 and linear drag (the workloads)
+    (4, 1.0, -1.0, 2e-2, 0.5),   # viscosity k^2 and large-scale drag k^-2
+    (2, 2.0, 0.0, 1e-3, 0.2),    # unsplit RK2
+])
+def test_step_factor_is_exp_minus_dt_dissipation(O, rko, alpnu, alpmu, nu, mu):
+    n, dt = 32, 0.05
+    o = mk(O, n, rko=rko, nu=nu, alpnu=alpnu, mu=mu, alpmu=alpmu, dt=dt, famp=0.0)
+    assert o.split == 0
+    z = _parallel_state(n)
+    (o.step4 if rko == 4 else o.step)(z)
+    zc = as_complex(z)
+    for kx, ky, a in PARALLEL_MODES:
+        k = math.hypot(kx, ky)
+        D = nu * k ** (2 * alpnu) + mu * k ** (2 * alpmu)
+        want = math.exp(-dt * D) * a
+        assert dt * D > 0.01  # the factor is far from 1, so a wrong exponent shows
+        assert abs(zc[ky % n, kx] - want) <= 1e-14 * abs(a), (kx, ky, zc[ky % n, kx], want)
+    # every other mode stays zero up to the rounding of the vanishing Jacobian
+    for kx, ky, _ in PARALLEL_MODES:
+        zc[ky % n, kx] = 0
+    assert np.abs(zc).max() < 1e-15
+
+
+def test_split_step_factor_is_exp_minus_dt_dissipation(O):
+    """The split RK2 factor emkx(i)*emky(j) (src/defk.f90:58-68, used when
+    alpnu = 1 and alpmu = 0) is exp(-dt/2 nu kx^2) * exp(-dt/2 (nu ky^2 + mu)),
+    whose square is exp(-dt (nu k^2 + mu))."""
+    n, dt, nu, mu = 32, 0.05, 2e-2, 0.7
+    o = mk(O, n, rko=2, nu=nu, alpnu=1.0, mu=mu, alpmu=0.0, dt=dt, famp=0.0)
+    assert o.split == 1
+    z = _parallel_state(n)
+    o.step(z)
+    zc = as_complex(z)
+    for kx, ky, a in PARALLEL_MODES:
+        want = math.exp(-dt * (nu * (kx * kx + ky * ky) + mu)) * a
+        assert abs(zc[ky % n, kx] - want) <= 1e-14 * abs(a)
+
+
+def test_step_factor_in_a_larger_box(O):
+    """xlx = 4 pi: wave numbers are multiples of dk = 1/2 (src/defk.f90:18-30),
+    so index (i, j) is the wave vector (i/2, j/2)."""
+    n, dt, nu, mu = 32, 0.05, 1e-2, 0.3
+    o = mk(O, n, rko=4, nu=nu, alpnu=2.0, mu=mu, alpmu=0.0, dt=dt, famp=0.0,
+           xlx=4 * np.pi)
+    assert o.dkx == 0.5 and o.dky == 0.5
+    z = _parallel_state(n)
+    o.step4(z)
+    zc = as_complex(z)
+    for kx, ky, a in PARALLEL_MODES:
+        k2 = (kx / 2) ** 2 + (ky / 2) ** 2
+        want = math.exp(-dt * (nu * k2 ** 2 + mu)) * a
+        assert abs(zc[ky % n, kx] - want) <= 1e-14 * abs(a)
+
+
+# --------------------------------------------------------------------------
 # diagnostics: src/diagnostics.f90
 # --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("xlx", [2 * np.pi, 4 * np.pi])
+@pytest.mark.parametrize("alpv", [1.0, 2.0])
+def test_flux_is_the_derivative_of_the_spectrum(O, xlx, alpv):
+    """fluxes (diagnostics.f90:84-109) against spectrum (:25-44): the
+    per-shell transfer flzm(b) = sum fac Re(conj(z) N) is dk times the
+    derivative of sz(z + eps N) at eps = 0, and flem likewise of sp; both
+    spectra are quadratic in z, so a central difference is exact up to
+    rounding for any eps.  This ties the flux weight (2 inside, 1 on the
+    kx = 0 and kx = N/2 columns) and the missing /dk to the spectrum's
+    weights, which the single-mode tests pin."""
+    n = 64
+    o = mk(O, n, alpv=alpv, xlx=xlx)
+    dk = o.dkx
+    z = random_spectral(n, 31)
+    znl = z.copy()
+    o.nlt(znl)
+    eps = 0.5
+    spp, szp = o.spectrum(z + eps * znl)
+    spm, szm = o.spectrum(z - eps * znl)
+    flzm = dk * (szp - szm) / (2 * eps)
+    flem = dk * (spp - spm) / (2 * eps)
+    fle, flz = o.fluxes(z, znl)
+    want_z = np.cumsum(flzm[::-1])[::-1]
+    want_e = np.cumsum(flem[::-1])[::-1]
+    assert np.abs(flz - want_z).max() <= 1e-12 * np.abs(want_z).max()
+    assert np.abs(fle - want_e).max() <= 1e-12 * np.abs(want_e).max()
+
+
+def test_binning_in_a_larger_box(O):
+    """xlx = 4 pi, dk = 1/2: a mode of wave vector (3, 4) (index (6, 8)) has
+    |k| = 5, shell int(5/dk)+1 = 11 (diagnostics.f90:19,31-34), and the
+    spectrum divides by dk (:42-44)."""
+    n = 64
+    o = mk(O, n, alpv=2.0, xlx=4 * np.pi)
+    assert o.dkx == 0.5
+    z = np.zeros((n, n + 2))
+    as_complex(z)[8, 6] = 0.3 + 0.4j            # |q|^2 = 1/4, k^2 = 25
+    sp, sz = o.spectrum(z)
+    assert np.count_nonzero(sz) == 1 and np.count_nonzero(sp) == 1
+    assert sz[10] == pytest.approx(0.25 / 0.5, rel=1e-15)
+    assert sp[10] == pytest.approx(0.25 / 25.0 / 0.5, rel=1e-15)
+    as_complex(z)[8, 6] = 0
+    as_complex(z)[n - 3, 0] = 2.0               # kx = 0, ky = -3/2: weight 1/2, shell 4
+    sp, sz = o.spectrum(z)
+    assert np.count_nonzero(sz) == 1 and sz[3] == pytest.approx(0.5 * 4.0 / 0.5)
+    # shell index of every mode: int(|k|/dk)+1 in grid-index units
+    ib = o.shell_index()
+    nh = n // 2 + 1
+    for j in range(n):
+        ky = j if j <= n // 2 else j - n
+        for i in range(nh):
+            if i or j:
+                assert ib[j, i] == math.isqrt(i * i + ky * ky) + 1
+    # a box of 8 pi (dk = 1/4) bins the same index pair into the same shell
+    o4 = mk(O, n, alpv=2.0, xlx=8 * np.pi)
+    assert np.array_equal(o4.shell_index(), ib)
+
 
 def test_spectrum_single_mode(O):
     n = 64
