@@ -6,23 +6,37 @@ initial field from seed 2 with spectrum k^4 exp(-(k/32)^2)).
 A step is what the reference's driver loop does per iteration when no
 diagnostics or output are due (src/main.f90:97-137): step4 + forcing, on a
 state resident in HBM.  W warm-up steps, then K timed steps bracketed by
-device synchronisation.  The roofline entry is the kernel class with the
-largest time per step, priced at its algorithmic bytes (the path's fixed
-byte model, `model_bytes`) over its average
-duration measured live over the timed region with HIP events on the
-library's own stream (only that class is event-timed there: every timed
-launch records two events, which costs a step a few per cent).  The
-per-class breakdown ("kernels") comes from a separate sample of
---breakdown-steps steps that times every class.
-The CPU baseline is the oracle (plain C, OpenMP) timed on this host on a
-bounded sample of the same workload.
+device synchronisation.
+
+The roofline entry is the kernel class with the largest time per step,
+priced at its algorithmic bytes: the bytes the pass must move on the
+compact layout the kernels keep between boundary crossings (every spectral
+field of the step is zero outside the 2/3 rectangle, so only that
+rectangle is read and written; `compact_bytes`), over its average duration
+measured live over the timed region with HIP events on the library's own
+stream (only that class is event-timed there: every timed launch records
+two events, which costs a step a few per cent).  Beside it: the HBM
+traffic the counters measured for that kernel (FETCH_SIZE + WRITE_SIZE,
+scaled by a plain copy, from the committed counter summary) as
+`frac_measured`, and the survey's unfused byte model (A = 8 N^2 per field
+pass, 468 B per grid-point-step; `model_bytes`) as `model_*`, which counts
+bytes the fused kernels never move and so is not a rate of moved bytes.
+The per-class breakdown ("kernels") comes from a separate sample of
+--breakdown-steps steps that times every class; a rate of moved bytes
+above the HBM peak stops the benchmark with an error.
+
+The CPU baseline is the oracle (plain C, OpenMP) timed on this host on
+bounded samples of the same workload, at every core the process may use
+and at one core, with the 256^2 Euler workload (1000 steps) beside it; the
+reference reports the mean wall time per iteration (src/main.f90:190-193).
 
 Also measured, in the same line: the RK2 variant of the headline workload
-("rk2"); the batched ensemble ("ensemble": 32 x 256^2, 24 x 512^2,
-8 x 1024^2, alpha cycling through 1/2, 1, 2) against the plain per-run loop
-of the same members, both over the same number of steps; and the cost of
-one adjoint gradient against the plain forward run ("adjoint": N=512,
-alpha=1, 200 RK4 steps, checkpoints every 20).
+("rk2"); the 256^2 Euler and 1024^2 SQG workloads' step loops ("configs");
+the batched ensemble ("ensemble": 32 x 256^2, 24 x 512^2, 8 x 1024^2,
+alpha cycling through 1/2, 1, 2) against the plain per-run loop of the same
+members, both over the same number of steps; and the cost of one adjoint
+gradient against the plain forward run ("adjoint": N=512, alpha=1, 200 RK4
+steps, checkpoints every 20).
 
 Prints ONE JSON line.
 """
@@ -60,7 +74,9 @@ def model_bytes(n, rko):
     else:  # RK2 27A: y start 4A, 2 x 5A, stage 1 2A + 2.5A + 4A, final 2A + 2.5A
         stage = 2 * a + 2.5 * a + 4 * a
         final = 2 * a + 2.5 * a
-    return {"y_start": 4 * a, "y_stage": stage, "y_final": final, "x_pass": 5 * a}
+    # the fused step boundary (final stage + forcing + next y start)
+    return {"y_start": 4 * a, "y_stage": stage, "y_final": final, "x_pass": 5 * a,
+            "y_bound": final + 4 * a}
 
 
 def compact_bytes(n, rect, rko):
@@ -79,8 +95,11 @@ def compact_bytes(n, rect, rko):
     else:
         stage = 5 * Y + E + Z
         final = 2 * Y + E + 2 * Z
+    # fused step boundary: the final stage's reads, z written once (forced),
+    # the next stage input's three fields written
+    bound = final + 3 * Y
     return {"y_start": Z + 3 * Y, "y_stage": stage, "y_final": final,
-            "x_pass": 5 * Y}
+            "x_pass": 5 * Y, "y_bound": bound}
 
 
 def read_traffic(kernel):
@@ -98,41 +117,86 @@ def read_traffic(kernel):
     return vals.get(kernel)
 
 
-def cpu_baseline(p, w0, seed, budget_s=10.0, max_steps=40):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """The cores this process may use: OMP_NUM_THREADS when set (the host's
+    share of cores for one GPU), else the CPU affinity set."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_rate(p, w0, seed, threads, budget_s, max_steps):
+    """Oracle step loop (step + forcing, no diagnostics or output) on
+    `threads` OpenMP threads: steps run until max_steps or until budget_s
+    has passed (at least one)."""
     import oracle
-    oracle.build()
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    from gturbo_amd import io
     oracle.set_threads(threads)
     o = oracle.Oracle(p)
-    from gturbo_amd import io
     z = io.field_to_ref(w0)
+    work = [o.field() for _ in range(4)]
     o.fft_dir(z)
     o.trunc(z)
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps:
         if p.rko == 4:
-            o.step4(z)
+            o.step4(z, *work)
         else:
-            o.step(z)
+            o.step(z, work[0], work[2], work[3])
         seed = o.forcing(z, seed)
         steps += 1
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
     return {"value": p.n * p.n * steps / el, "unit": "grid-pt-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{steps} step(s)+forcing of the same {p.n}^2 RK{p.rko} workload "
-                      f"({el:.1f} s wall, {threads} OpenMP threads)"}
+            "ms_per_step": el / steps * 1e3,
+            "sample": f"{steps} step(s)+forcing of {p.n}^2 RK{p.rko} ({el:.1f} s wall, "
+                      f"{threads} OpenMP thread{'s' if threads > 1 else ''})"}
 
 
-def rk_rate(rko, n, steps, warmup):
-    """grid-pt-steps/s of step(+forcing) for the headline workload at RK order
-    rko (the same loop as the headline measurement)."""
+def cpu_baseline(p, w0, seed):
+    """The oracle on this host's cores: the headline workload at every core
+    this process may use (`value`) and at one core, and the 256^2 Euler
+    workload (all 1000 steps at every core; a bounded sample at one core)."""
+    import oracle
+    from gturbo_amd import ic, workloads
+    oracle.build()
+    nt = cpu_threads()
+    full = cpu_rate(p, w0, seed, nt, 8.0, 40)
+    one = cpu_rate(p, w0, seed, 1, 6.0, 40)
+    w = workloads()["euler256"]
+    w256 = ic.physical_field(w.params, w.ic)
+    e_all = cpu_rate(w.params, w256, w.forcing_seed, nt, 15.0, w.params.npas)
+    e_one = cpu_rate(w.params, w256, w.forcing_seed, 1, 5.0, w.params.npas)
+    oracle.set_threads(nt)
+    return {"value": full["value"], "unit": "grid-pt-steps/s", "cores": nt, "kind": "port",
+            "sample": full["sample"], "ms_per_step": full["ms_per_step"],
+            "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "cpu_model": cpu_model(),
+            "one_core": one,
+            "euler256": {"workload": w.name, "all_cores": e_all, "one_core": e_one}}
+
+
+def workload_rate(name, steps, warmup, n=None):
+    """grid-pt-steps/s of the step loop (step or step4, then forcing; no
+    diagnostics or output) of a named workload on the GPU."""
     from gturbo_amd import ic, io, workloads
     from gturbo_amd.solver import Solver
-    w = workloads()[f"euler4096_rk{rko}"]
-    p = w.params.replace(n=n)
+    w = workloads()[name]
+    p = w.params if n is None else w.params.replace(n=n)
     with Solver(p) as s:
         z = io.field_to_ref(ic.physical_field(p, w.ic))
         arrs = [z] + [s.field() for _ in range(4)]
@@ -146,7 +210,7 @@ def rk_rate(rko, n, steps, warmup):
             if it == warmup:
                 s.synchronize()
                 t0 = time.perf_counter()
-            if rko == 4:
+            if p.rko == 4:
                 s.step4(*arrs)
             else:
                 s.step(arrs[0], arrs[1], arrs[3], arrs[4])
@@ -155,7 +219,7 @@ def rk_rate(rko, n, steps, warmup):
         el = time.perf_counter() - t0
         s.update_host(z)
         ok = bool(np.all(np.isfinite(z)))
-    return {"workload": w.name, "value": n * n * steps / el, "unit": "grid-pt-steps/s",
+    return {"workload": w.name, "value": p.n * p.n * steps / el, "unit": "grid-pt-steps/s",
             "ms_per_step": el / steps * 1e3, "steps": steps, "finite": ok}
 
 
@@ -216,6 +280,43 @@ def ensemble_bench(steps):
             "finite": finite}
 
 
+def diag_bench(sizes=(1024, 4096), reps=20):
+    """Wall time per call of the diagnostics entry points (spectrum, fluxes,
+    glob; src/diagnostics.f90) on the spectral state of the 1024^2 SQG and
+    4096^2 Euler workloads after fft_dir + trunc, results copied to the
+    host; the nonlinear term the fluxes need is computed once, outside the
+    timing."""
+    from gturbo_amd import ic, io, workloads
+    from gturbo_amd.solver import Solver
+    names = {1024: "sqg1024", 4096: "euler4096_rk4"}
+    out = {}
+    for n in sizes:
+        w = workloads()[names[n]]
+        p = w.params
+        with Solver(p) as s:
+            z = io.field_to_ref(ic.physical_field(p, w.ic))
+            znl, u, v = s.field(), s.field(), s.field()
+            for a in (z, znl, u, v):
+                s.enter_data(a)
+            s.fft_dir(z)
+            s.trunc(z)
+            s.copy(znl, z)
+            s.nlt(znl, u, v)
+            res = {}
+            for name, call in (("spectrum", lambda: s.spectrum(z)),
+                               ("fluxes", lambda: s.fluxes(z, znl)),
+                               ("glob", lambda: s.glob(z))):
+                call()  # first call allocates the partial sums
+                ts = []
+                for _ in range(reps):
+                    t0 = time.perf_counter()
+                    call()
+                    ts.append(time.perf_counter() - t0)
+                res[name + "_ms"] = float(np.median(ts)) * 1e3
+            out[f"{n}"] = {"workload": w.name, **res}
+    return out
+
+
 def adjoint_bench():
     """The adjoint workload (N=512, alpha=1, RK4, dt=5e-4, 200 steps, forcing
     off, checkpoints every 20 steps): time of gt_adj_forward + gt_adj_backward
@@ -274,12 +375,19 @@ def main():
                     help="steps of the RK2 variant of the headline workload (0: skip)")
     ap.add_argument("--ens-steps", type=int, default=100,
                     help="steps of the batched-ensemble comparison (0: skip)")
+    ap.add_argument("--no-configs", dest="config_steps", action="store_false",
+                    help="skip the 256^2 and 1024^2 workloads' step loops")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one device-to-device copy of a reference-layout "
                          "field (a plain copy of known size for HBM counter scaling)")
+    ap.add_argument("--diag-only", action="store_true",
+                    help="only time the diagnostics entry points at 1024^2 and 4096^2")
     args = ap.parse_args()
+    if args.diag_only and args.gpus == 1:
+        print(json.dumps({"diagnostics": diag_bench()}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "gturbo_amd runs on one GPU: the 2D pseudospectral "
                           "step is not sharded across GPUs; "
@@ -326,6 +434,10 @@ def main():
     ab = model_bytes(p.n, p.rko)
     cb = compact_bytes(p.n, s.rect, p.rko)
 
+    def traffic_of(name):
+        # the committed counter summary is of the 4096^2 RK4 workload only
+        return read_traffic(name) if (args.n == 4096 and p.rko == 4) else None
+
     # Kernel breakdown: a sample of its own with every kernel class
     # event-timed (each timed launch records two events on the stream,
     # which costs a step a few per cent).
@@ -343,11 +455,16 @@ def main():
         avg_s = ms / cnt / 1e3
         ent = {"launches_per_step": cnt / args.breakdown_steps, "avg_us": avg_s * 1e6,
                "ms_per_step": ms / args.breakdown_steps}
-        if name in ab:
-            ent["algorithmic_bytes"] = ab[name]
-            ent["achieved_GBs"] = ab[name] / avg_s / 1e9
-            ent["compact_bytes"] = cb[name]
-            ent["compact_GBs"] = cb[name] / avg_s / 1e9
+        if name in cb:
+            # moved bytes: what the pass reads and writes on the compact layout,
+            # and what the counters measured (4096^2 RK4 only)
+            ent["algorithmic_bytes"] = cb[name]
+            ent["achieved_GBs"] = cb[name] / avg_s / 1e9
+            tr = traffic_of(name)
+            if tr is not None:
+                ent["measured_bytes"] = tr
+                ent["measured_GBs"] = tr / avg_s / 1e9
+            ent["model_bytes"] = ab[name]
         kernels[name] = ent
     dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["ms_per_step"])
 
@@ -369,15 +486,23 @@ def main():
 
     rate = p.n * p.n * args.steps / el
     dom_avg_s = dom_ms / dom_cnt / 1e3
-    ach = ab[dom] / dom_avg_s / 1e9
-    # the committed counter summary is of the 4096^2 RK4 workload only
-    traffic = read_traffic(dom) if (args.n == 4096 and p.rko == 4) else None
+    ach = cb[dom] / dom_avg_s / 1e9
+    traffic = traffic_of(dom)
     roof = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-            "avg_us": dom_avg_s * 1e6, "launches": dom_cnt,
-            # the bytes the compact layout moves instead, and the measured HBM rate
-            "compact_bytes": cb[dom], "achieved_compact": cb[dom] / dom_avg_s / 1e9,
-            "hbm_measured": None if traffic is None else traffic / dom_avg_s / 1e9}
+            "algorithmic_bytes": cb[dom], "avg_us": dom_avg_s * 1e6, "launches": dom_cnt,
+            "achieved_measured": None if traffic is None else traffic / dom_avg_s / 1e9,
+            "frac_measured": None if traffic is None else traffic / dom_avg_s / 1e9 / HBM_PEAK_GBS,
+            # the survey's unfused model: bytes an unfused step would move
+            "model_bytes": ab[dom], "model_achieved": ab[dom] / dom_avg_s / 1e9,
+            "model_frac": ab[dom] / dom_avg_s / 1e9 / HBM_PEAK_GBS}
+    over = [(k, v[f]) for k, v in kernels.items() for f in ("achieved_GBs", "measured_GBs")
+            if f in v and v[f] > HBM_PEAK_GBS]
+    if roof["achieved"] > HBM_PEAK_GBS or (roof["achieved_measured"] or 0) > HBM_PEAK_GBS:
+        over.append((dom, roof["achieved"]))
+    if over:
+        print(json.dumps({"error": f"moved-byte rates above the HBM peak: {over}"}))
+        sys.exit(4)
     step_bytes = sum(ab[k] * kernels[k]["launches_per_step"] for k in ab if k in kernels)
     step_compact = sum(cb[k] * kernels[k]["launches_per_step"] for k in cb if k in kernels)
     out = {
@@ -397,24 +522,31 @@ def main():
                    "n": p.n, "rko": p.rko, "dt": p.dt, "alpv": p.alpv,
                    "kept_rect": list(s.rect), "nf": s.nf},
         "roofline": roof,
-        "step_model": {"algorithmic_bytes_per_step": step_bytes,
-                       "bytes_per_gp_step": step_bytes / (p.n * p.n),
-                       "GBs_at_measured_rate": step_bytes * args.steps / el / 1e9,
-                       "frac": step_bytes * args.steps / el / 1e9 / HBM_PEAK_GBS,
-                       "compact_bytes_per_step": step_compact,
-                       "compact_bytes_per_gp_step": step_compact / (p.n * p.n)},
+        "step": {"algorithmic_bytes_per_step": step_compact,
+                 "bytes_per_gp_step": step_compact / (p.n * p.n),
+                 "GBs_at_measured_rate": step_compact * args.steps / el / 1e9,
+                 "frac": step_compact * args.steps / el / 1e9 / HBM_PEAK_GBS,
+                 "model_bytes_per_step": step_bytes,
+                 "model_bytes_per_gp_step": step_bytes / (p.n * p.n),
+                 "model_frac": step_bytes * args.steps / el / 1e9 / HBM_PEAK_GBS},
         "kernels": kernels,
         "kernels_sample_steps": args.breakdown_steps,
     }
     s.close()
     if args.rk2_steps > 0 and args.rko == 4:
-        out["rk2"] = rk_rate(2, args.n, args.rk2_steps, 5)
+        out["rk2"] = workload_rate("euler4096_rk2", args.rk2_steps, 5, n=args.n)
+    if args.config_steps:
+        out["configs"] = {"euler256": workload_rate("euler256", 1000, 10),
+                          "sqg1024": workload_rate("sqg1024", 2000, 10)}
     if args.ens_steps > 0:
         out["ensemble"] = ensemble_bench(args.ens_steps)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench()
+    if args.config_steps and args.n == 4096:
+        out["diagnostics"] = diag_bench()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, w0, w.forcing_seed)
+        out["cpu_baseline"]["gpu_over_cpu"] = rate / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 
 

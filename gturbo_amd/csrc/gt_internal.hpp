@@ -32,6 +32,14 @@ struct Phys {
   double nu, mu, alpnu, alpmu, alpv;
 };
 
+// One forcing update of a kx column for the fused step boundary: the mode
+// (or Hermitian partner, conj = 1) in compact row `row`, phase k of the
+// step's draws, weights ff (src/forcing.f90:25-31).
+struct FEntry {
+  int row, k, conj;
+  double ff1, ff2;
+};
+
 // Arguments of the column (y) and row (x) passes of the nonlinear term.
 struct PassArgs {
   Rect rc;
@@ -51,6 +59,13 @@ struct PassArgs {
   // batched ensemble passes only: x-pass packing (0 = one row per line,
   // 1 = row pairs)
   int xvar;
+  // fused step boundary (C_RK4_4S, C_RK2_2S) with forcing: column c's updates
+  // are fce[fcs[c] .. fcs[c+1]-1], with the step's phases fph (cos, sin) and
+  // amplitude famp = sqrt(dt)*famp; fcs = nullptr: no forcing
+  const int* fcs;
+  const FEntry* fce;
+  const double2* fph;
+  double famp;
 };
 
 // Arguments of the adjoint passes (transposed nonlinear term inside the
@@ -90,6 +105,10 @@ enum CMode : int {
   C_RK4_4 = 5,
   C_RK2_1 = 6,
   C_RK2_2 = 7,
+  // step boundary: the last stage of a step (C_RK4_4 / C_RK2_2), the step's
+  // forcing, and the inverse part of the next step (C_START), in one pass
+  C_RK4_4S = 8,
+  C_RK2_2S = 9,
 };
 
 // Column-tile width of the y pass (BT), y rows per workgroup of the x pass,
@@ -154,6 +173,9 @@ struct DiagArgs {
   int kyp;
   Phys ph;
   const double2* z;
+  // glob only: mode (column c, row r) of z at c * zcs + r * zrs -- (kyp, 1)
+  // on the compact layout, (1, N/2+1) on the reference layout (full rect)
+  int zcs, zrs;
   const double2* znl;
   double* part1;  // (kx+1) x NBIN partial sums
   double* part2;

@@ -57,6 +57,25 @@ struct RectTables {
   int kyp = 0;
   double* emk = nullptr;
   ForcedMode* modes = nullptr;
+  // the forced modes by kx column (fused step boundary): column c's updates
+  // are fce[fcs[c] .. fcs[c+1]-1]
+  int* fcs = nullptr;
+  gt::FEntry* fce = nullptr;
+};
+
+// The last stage of a step left for the next step to fuse with its first
+// pass (the driver's step, forcing, step sequence): kind GT_PEND_RK4 or
+// GT_PEND_RK2, the state and accumulator arrays and the pass arguments; with
+// forcing = true the step's forcing draws are already in slot `slot`.
+enum : int { PEND_NONE = 0, PEND_RK4 = 1, PEND_RK2 = 2 };
+struct Pending {
+  int kind = PEND_NONE;
+  Mirror* z = nullptr;
+  Mirror* znew = nullptr;
+  RectTables* t = nullptr;
+  gt::PassArgs a{};
+  bool forced = false;
+  unsigned slot = 0;
 };
 
 struct TimedLaunch {
@@ -110,6 +129,8 @@ struct gt_ctx {
   std::unordered_map<const void*, gtr::Mirror> present;
 
   AdjState* adj = nullptr;  // checkpoints and buffers of gt_adj_*
+
+  gtr::Pending pend;  // deferred end of the last step (fused into the next)
 };
 
 namespace gtr {
@@ -131,6 +152,9 @@ int r_pass(gt_ctx* c, const gt::PassArgs& a);
 void forcing_phases(int nf, int32_t* seed, double2* out);
 int forcing_on(gt_ctx* c, RectTables* t, double2* z, int32_t* seed);
 void adj_free(gt_ctx* c);  // gturbo_adjoint.cpp
+// Runs the deferred end of the last step (its last stage and forcing), if
+// any: every entry point that reads or writes device data calls it first.
+int flush_pending(gt_ctx* c);
 
 // roctx range over one C entry point (rocprofv3 --marker-trace shows them;
 // without a profiler attached they cost a call each).

@@ -234,6 +234,7 @@ int gt_adj_forward(gt_ctx* c, const double* w0, double* wT, int32_t nsteps, int3
                    int32_t* seed) {
   GT_RANGE();
   CHK(adj_check(c));
+  CHK(flush_pending(c));
   if (!seed) return fail(GT_ERR_ARG, "null seed");
   if (nsteps < 1 || kchk < 1 || kchk > nsteps)
     return fail(GT_ERR_ARG, "need 1 <= kchk <= nsteps (got %d, %d)", kchk, nsteps);
@@ -265,6 +266,7 @@ int gt_adj_forward(gt_ctx* c, const double* w0, double* wT, int32_t nsteps, int3
 int gt_adj_backward(gt_ctx* c, const double* gT, double* g0) {
   GT_RANGE();
   CHK(adj_check(c));
+  CHK(flush_pending(c));
   AdjState* s = c->adj;
   if (!s || !s->ready) return fail(GT_ERR_ARG, "gt_adj_backward needs a gt_adj_forward first");
   Mirror *mT, *m0;

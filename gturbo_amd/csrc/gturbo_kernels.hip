@@ -114,6 +114,23 @@ __device__ __forceinline__ void line_map(int tid, int& line, int& j) {
 // y pass
 // ---------------------------------------------------------------------------
 
+// The forcing of compact row `row` of column c (fused step boundary): the
+// same operations as forcing_k's force_one on that element, in the order of
+// the forced-mode table.
+__device__ __forceinline__ void force_row(const PassArgs& a, int c, int row, double2& z) {
+#pragma clang fp contract(off)
+  if (a.fcs == nullptr) return;
+  const int e1 = a.fcs[c + 1];
+  for (int e = a.fcs[c]; e < e1; ++e) {
+    const FEntry f = a.fce[e];
+    if (f.row != row) continue;
+    const double2 cs = a.fph[f.k];
+    const double a1 = a.famp * f.ff1 * cs.x, a2 = a.famp * f.ff2 * cs.y;
+    z.x = z.x + a1;
+    z.y = f.conj ? z.y - a2 : z.y + a2;
+  }
+}
+
 // One tile of BT kx-columns; lds holds BT padded lines.  SCHED is the radix
 // order of the transforms (fft_reg).
 // IP (4096-point lines only): the y buffers hold the rows of a column in
@@ -132,7 +149,8 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
                 "in-place y pass: two 4096-point lines or one 8192-point line");
   constexpr bool FWD = MODE != C_START;
   constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
-                       MODE == C_RK4_3 || MODE == C_RK2_1;
+                       MODE == C_RK4_3 || MODE == C_RK2_1 || MODE == C_RK4_4S ||
+                       MODE == C_RK2_2S;
   int b, j;
   if constexpr (IP && BT == 2) {
     b = (tid >> 6) & 1;
@@ -226,6 +244,23 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
           double2 zz = make_double2(zn.x + ph.dt6 * k.x, zn.y + ph.dt6 * k.y);
           if (mean) zz = make_double2(0.0, 0.0);
           a.z[q] = zz;
+        } else if constexpr (MODE == C_RK4_4S) {
+          // rkstep.f90:123,127, then forcing.f90:16-33 on this mode
+          const double2 zn = a.znew[q];
+          double2 zz = make_double2(zn.x + ph.dt6 * k.x, zn.y + ph.dt6 * k.y);
+          if (mean) zz = make_double2(0.0, 0.0);
+          force_row(a, c, row, zz);
+          a.z[q] = zz;
+          s[r] = zz;  // the next step's stage input
+        } else if constexpr (MODE == C_RK2_2S) {
+          // rkstep.f90:54/64 and :72, then forcing.f90:16-33 on this mode
+          const double E = a.emk[q];
+          const double2 zz = a.z[q];
+          double2 zo = make_double2(E * (E * zz.x + ph.dt * k.x), E * (E * zz.y + ph.dt * k.y));
+          if (mean) zo = make_double2(0.0, 0.0);
+          force_row(a, c, row, zo);
+          a.z[q] = zo;
+          s[r] = zo;
         } else if constexpr (MODE == C_RK2_1) {
           // rkstep.f90:24/34 and :42
           const double E = a.emk[q];
@@ -956,6 +991,8 @@ static hipError_t c_pass_launch_ak(int mode, const PassArgs& a, hipStream_t s) {
     case C_RK4_4: c_pass<L, C_RK4_4, AK, TR><<<g, b, 0, s>>>(a); break;
     case C_RK2_1: c_pass<L, C_RK2_1, AK, TR><<<g, b, 0, s>>>(a); break;
     case C_RK2_2: c_pass<L, C_RK2_2, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK4_4S: c_pass<L, C_RK4_4S, AK, TR><<<g, b, 0, s>>>(a); break;
+    case C_RK2_2S: c_pass<L, C_RK2_2S, AK, TR><<<g, b, 0, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1341,101 +1378,188 @@ __global__ void ens_forcing_k(const EnsForce* __restrict__ f, int nmem, int tota
 __global__ void ens_tick_k(int* step) { step[0] = step[0] + 1; }
 
 // ---------------------------------------------------------------------------
-// Diagnostics (src/diagnostics.f90), deterministic: per column the modes of a
-// shell form one run of rows in each half of the column (the shell index is
-// monotone in |ky|); runs are summed in row order into part[c][b], and the
-// columns are then summed in order of c.
+// Diagnostics (src/diagnostics.f90), deterministic, in two launches each.
+//
+// Spectrum and fluxes: one workgroup (4 waves) per kx column bins the
+// column's modes into LDS bins, one per shell.  In each half of a column
+// (ky >= 0: rows 0..p; ky < 0: rows p+1..) the shell index is monotone in
+// the row, so a chunk of 64 consecutive rows of one wave holds runs of equal
+// shells: a segmented wavefront scan (lane shuffles, a fixed tree) sums each
+// run, the run's last lane carries the sum, and runs continued across the
+// four waves of a 256-row chunk are joined in wave order by the run's first
+// lane, which alone adds it to the shell's LDS bin.  The column's bins then
+// go to part[c][b].  A second launch sums part[.][b] over the columns, one
+// wavefront per shell: lanes take columns c = lane, lane + 64, ... in
+// order, then a fixed shuffle tree.  The order of every sum is fixed, so
+// the result does not change from run to run; it replaces the reference's
+// unprotected accumulations on the GPU (diagnostics.f90:25-39,84-101).
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ int shell_of(const Phys& ph, double fkxs, int p) {
+__host__ __device__ __forceinline__ int shell_of(const Phys& ph, double fkxs, int p) {
   const double fky = (double)kyi_of(p, ph.n) * ph.dky;
   const double k = sqrt(fkxs + fky * fky);
   return (int)(k / ph.dkx) + 1;  // diagnostics.f90:32-34 (dk = dkx)
 }
 
-// first row r in [lo, hi) with pred(r) true, pred monotone false->true
-template <class P>
-__device__ __forceinline__ int lower_row(int lo, int hi, P pred) {
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (pred(mid)) hi = mid;
-    else lo = mid + 1;
+// Shells a column's kept rows fall in: [lo, hi], hi below NBIN
+// (diagnostics.f90:35 bins only ib < NBIN).
+__host__ __device__ __forceinline__ void column_shells(const Phys& ph, const Rect& rc, int c,
+                                                       int& lo, int& hi) {
+  const int n = ph.n;
+  const double fkx = (double)c * ph.dkx, fkxs = fkx * fkx;
+  lo = shell_of(ph, fkxs, 0);
+  hi = shell_of(ph, fkxs, fft_of(rc.p, rc, n));
+  if (rc.ky() > rc.p + 1) {
+    const int h2 = shell_of(ph, fkxs, fft_of(rc.p + 1, rc, n));
+    hi = h2 > hi ? h2 : hi;
   }
-  return lo;
+  hi = hi < n / 2 - 1 ? hi : n / 2 - 1;
+}
+
+constexpr int kDiagT = 256;  // threads of a diagnostics workgroup (4 waves)
+
+// Inclusive sums of v1, v2 over each run of equal keys in the wave (runs
+// are contiguous); returns true on a run's last lane.
+__device__ __forceinline__ bool wave_runs(int key, double& v1, double& v2) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int kprev = __shfl_up(key, 1);
+  const bool head = lane == 0 || kprev != key;
+  const unsigned long long heads = __ballot(head);
+  // run number: heads at or below this lane
+  const int run = __popcll(heads & (~0ull >> (63 - lane)));
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const double t1 = __shfl_up(v1, d), t2 = __shfl_up(v2, d);
+    const int rr = __shfl_up(run, d);
+    if (lane >= d && rr == run) {
+      v1 = v1 + t1;
+      v2 = v2 + t2;
+    }
+  }
+  const bool next_head = lane == 63 ? true : ((heads >> (lane + 1)) & 1ull) != 0;
+  return next_head;
 }
 
 // which = 0: spectrum (sp, sz);  which = 1: fluxes (flzm, flem)
-__global__ void __launch_bounds__(256) shell_part_k(DiagArgs d, int which) {
+// nbl: the most shells of any column (LDS bins per sum).
+__global__ void __launch_bounds__(kDiagT) shell_bin_k(DiagArgs d, int which, int nbl) {
+  extern __shared__ double dsm[];
   const Phys& ph = d.ph;
   const Rect rc = d.rc;
-  const int n = ph.n, nbin = n / 2, c = blockIdx.x;
+  const int n = ph.n, nbin = n / 2, c = blockIdx.x, tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  double* bin1 = dsm;                  // shell b of the column in bin b - lo
+  double* bin2 = bin1 + nbl;
+  double* sv1 = bin2 + nbl;            // per-lane run sums of a chunk
+  double* sv2 = sv1 + kDiagT;
+  int* skey = reinterpret_cast<int*>(sv2 + kDiagT);  // key on a run's last lane, else -1
+  int* ftail = skey + kDiagT;                        // each wave's first run end
   const double fkx = (double)c * ph.dkx, fkxs = fkx * fkx;
-  const int P = rc.p, Ky = rc.ky();
-  auto ibr = [&](int r) { return shell_of(ph, fkxs, fft_of(r, rc, n)); };
-  const int blo = ibr(0);
-  int bhi = ibr(P);
-  if (Ky > P + 1) bhi = max(bhi, ibr(P + 1));
-  bhi = min(bhi, nbin - 1);
-  const double edge = (c == 0 || c == n / 2) ? 1.0 : 0.0;
+  const bool edge = (c == 0 || c == n / 2);
   const size_t cq = (size_t)c * d.kyp;
-  for (int b = blo + (int)threadIdx.x; b <= bhi; b += blockDim.x) {
-    double acc1 = 0.0, acc2 = 0.0;
-    // positive half: rows 0..P, shell non-decreasing in r
-    const int p0 = lower_row(0, P + 1, [&](int r) { return ibr(r) >= b; });
-    const int p1 = lower_row(p0, P + 1, [&](int r) { return ibr(r) > b; });
-    // negative half: rows P+1..Ky-1, shell non-increasing in r
-    const int n0 = lower_row(P + 1, Ky, [&](int r) { return ibr(r) <= b; });
-    const int n1 = lower_row(n0, Ky, [&](int r) { return ibr(r) < b; });
-    for (int seg = 0; seg < 2; ++seg) {
-      const int r0 = seg ? n0 : p0, r1 = seg ? n1 : p1;
-      for (int r = r0; r < r1; ++r) {
-        if (c == 0 && r == 0) continue;  // diagnostics.f90:28
+  int blo, bhi;
+  column_shells(ph, rc, c, blo, bhi);
+  if (bhi - blo + 1 > nbl) bhi = blo + nbl - 1;  // never: nbl is the maximum
+  for (int b = blo + tid; b <= bhi; b += kDiagT) {
+    bin1[b - blo] = 0.0;
+    bin2[b - blo] = 0.0;
+  }
+  __syncthreads();
+  const int P = rc.p, Ky = rc.ky();
+  for (int half = 0; half < 2; ++half) {
+    const int r0 = half ? P + 1 : 0, r1 = half ? Ky : P + 1;  // rows [r0, r1)
+    for (int base = r0; base < r1; base += kDiagT) {
+      const int r = base + tid;
+      int key = -1;
+      double v1 = 0.0, v2 = 0.0;
+      if (r < r1 && !(c == 0 && r == 0)) {  // diagnostics.f90:28
         const int p = fft_of(r, rc, n);
         const double fky = (double)kyi_of(p, n) * ph.dky;
         const double k2 = fkxs + fky * fky;
-        const double ka = kalpha(ph, k2);
-        const double2 zv = d.z[cq + r];
-        if (which == 0) {
-          const double fac = edge != 0.0 ? 0.5 : 1.0;
-          const double am = hypot(zv.x, zv.y);
-          const double a2 = am * am;
-          acc1 = acc1 + fac * a2 / ka;  // diagnostics.f90:36-37
-          acc2 = acc2 + fac * a2;
-        } else {
-          const double fac = edge != 0.0 ? 1.0 : 2.0;
-          const double2 nv = d.znl[cq + r];
-          const double fz = zv.x * nv.x + zv.y * nv.y;  // diagnostics.f90:93
-          const double fe = fz / ka;
-          acc1 = acc1 + fac * fz;  // flzm, diagnostics.f90:98
-          acc2 = acc2 + fac * fe;  // flem
+        const double k = sqrt(k2);
+        const int ib = (int)(k / ph.dkx) + 1;  // diagnostics.f90:34
+        if (ib < nbin && ib >= blo && ib <= bhi) {
+          key = ib;
+          const double ka = kalpha(ph, k2);
+          const double2 zv = d.z[cq + r];
+          if (which == 0) {
+            const double fac = edge ? 0.5 : 1.0;  // diagnostics.f90:29-30
+            const double am = hypot(zv.x, zv.y);
+            const double a2 = am * am;
+            v1 = fac * a2 / ka;  // sp, diagnostics.f90:36
+            v2 = fac * a2;       // sz, :37
+          } else {
+            const double fac = edge ? 1.0 : 2.0;  // diagnostics.f90:89-90
+            const double2 nv = d.znl[cq + r];
+            const double fz = zv.x * nv.x + zv.y * nv.y;  // :93
+            v1 = fac * fz;          // flzm, :98
+            v2 = fac * (fz / ka);   // flem, :95,99
+          }
         }
       }
+      const bool last = wave_runs(key, v1, v2);
+      const unsigned long long lasts = __ballot(last);
+      skey[tid] = last ? key : -1;
+      sv1[tid] = v1;
+      sv2[tid] = v2;
+      if (lane == 0) ftail[wave] = (wave << 6) + __builtin_ctzll(lasts);
+      __syncthreads();
+      if (last && key >= 0) {
+        // A wave's first run continues the previous wave's last run when the
+        // keys agree: that run belongs to the lane that ends its first piece.
+        const bool cont = wave > 0 && tid == ftail[wave] && skey[(wave << 6) - 1] == key;
+        if (!cont) {
+          double a1 = v1, a2 = v2;
+          if (lane == 63) {
+            for (int w2 = wave + 1; w2 < kDiagT / 64; ++w2) {
+              const int t = ftail[w2];
+              if (skey[t] != key) break;
+              a1 = a1 + sv1[t];
+              a2 = a2 + sv2[t];
+              if ((t & 63) != 63) break;
+            }
+          }
+          bin1[key - blo] = bin1[key - blo] + a1;
+          bin2[key - blo] = bin2[key - blo] + a2;
+        }
+      }
+      __syncthreads();
     }
-    d.part1[(size_t)c * nbin + (b - 1)] = acc1;
-    d.part2[(size_t)c * nbin + (b - 1)] = acc2;
+  }
+  for (int b = blo + tid; b <= bhi; b += kDiagT) {
+    d.part1[(size_t)c * nbin + (b - 1)] = bin1[b - blo];
+    d.part2[(size_t)c * nbin + (b - 1)] = bin2[b - blo];
   }
 }
 
-__global__ void shell_reduce_k(DiagArgs d, int which) {
+// Sum of a wavefront's values in a fixed tree order; lane 0 holds the total.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off);
+  return v;
+}
+
+// One wavefront per shell: the columns' bins of shell b, over the columns in
+// a fixed order.
+__global__ void __launch_bounds__(kDiagT) shell_sum_k(DiagArgs d, int which) {
   const Phys& ph = d.ph;
   const Rect rc = d.rc;
   const int n = ph.n, nbin = n / 2;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x + 1;  // shell 1..NBIN
+  const int b = (int)((blockIdx.x * kDiagT + threadIdx.x) >> 6) + 1;  // shell 1..NBIN
+  const int lane = threadIdx.x & 63;
   if (b > nbin) return;
   double s1 = 0.0, s2 = 0.0;
-  const int P = rc.p, Ky = rc.ky();
-  for (int c = 0; c <= rc.kx; ++c) {
-    const double fkx = (double)c * ph.dkx, fkxs = fkx * fkx;
-    const int blo = shell_of(ph, fkxs, 0);
-    if (blo > b) break;  // shells only grow with c
-    int bhi = shell_of(ph, fkxs, fft_of(P, rc, n));
-    if (Ky > P + 1) bhi = max(bhi, shell_of(ph, fkxs, fft_of(P + 1, rc, n)));
-    bhi = min(bhi, nbin - 1);
-    if (b <= bhi) {
+  for (int c = lane; c <= rc.kx; c += 64) {
+    int lo, hi;
+    column_shells(ph, rc, c, lo, hi);
+    if (b >= lo && b <= hi) {
       s1 = s1 + d.part1[(size_t)c * nbin + (b - 1)];
       s2 = s2 + d.part2[(size_t)c * nbin + (b - 1)];
     }
   }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (lane != 0) return;
   if (which == 0) {  // diagnostics.f90:42-45
     d.out1[b - 1] = s1 / ph.dkx;
     d.out2[b - 1] = s2 / ph.dkx;
@@ -1461,49 +1585,72 @@ __global__ void suffix_k(DiagArgs d) {
   d.out2[i] = sz;  // flz
 }
 
-// src/diagnostics.f90:147-163, per column, then over columns in order.
-__global__ void __launch_bounds__(256) glob_part_k(DiagArgs d) {
+// k2**a for the dissipation exponents of glob: repeated products for the
+// small whole exponents the runs use (k^8 hyperviscosity is a = 4, linear
+// drag a = 0), pow otherwise.
+__device__ __forceinline__ double pow_k2(double k2, double a) {
+  if (a == 0.0) return 1.0;
+  if (a == 1.0) return k2;
+  if (a == 2.0) return k2 * k2;
+  if (a == 3.0) return (k2 * k2) * k2;
+  if (a == 4.0) {
+    const double t = k2 * k2;
+    return t * t;
+  }
+  return pow(k2, a);
+}
+
+// src/diagnostics.f90:147-163: one workgroup per column; each thread sums
+// its rows in order, then a fixed wavefront tree and the four waves in
+// order.  dise_* use k2**alpnu / ka for k2**(alpnu - alpv/2).
+__global__ void __launch_bounds__(kDiagT) glob_col_k(DiagArgs d) {
   const Phys& ph = d.ph;
   const Rect rc = d.rc;
   const int n = ph.n, c = blockIdx.x, Ky = rc.ky();
   const double fkx = (double)c * ph.dkx, fkxs = fkx * fkx;
   const double fac = (c == 0 || c == n / 2) ? 0.5 : 1.0;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int r = threadIdx.x; r < Ky; r += blockDim.x) {
+  for (int r = threadIdx.x; r < Ky; r += kDiagT) {
     if (c == 0 && r == 0) continue;
     const int p = fft_of(r, rc, n);
     const double fky = (double)kyi_of(p, n) * ph.dky;
     const double k2 = fkxs + fky * fky;
     const double ka = kalpha(ph, k2);
-    const double2 zv = d.z[(size_t)c * d.kyp + r];
+    const double2 zv = d.z[(size_t)c * d.zcs + (size_t)r * d.zrs];
     const double am = hypot(zv.x, zv.y);
     const double scra = am * am;
-    acc[0] = acc[0] + fac * scra / ka;
-    acc[1] = acc[1] + fac * scra;
-    acc[2] = acc[2] + 2.0 * fac * ph.nu * pow(k2, ph.alpnu) * scra;
-    acc[3] = acc[3] + 2.0 * fac * ph.mu * pow(k2, ph.alpmu) * scra;
-    acc[4] = acc[4] + 2.0 * fac * ph.nu * pow(k2, ph.alpnu - ph.alpv / 2.0) * scra;
-    acc[5] = acc[5] + 2.0 * fac * ph.mu * pow(k2, ph.alpmu - ph.alpv / 2.0) * scra;
+    const double dnu = 2.0 * fac * ph.nu * pow_k2(k2, ph.alpnu) * scra;
+    const double dmu = 2.0 * fac * ph.mu * pow_k2(k2, ph.alpmu) * scra;
+    acc[0] = acc[0] + fac * scra / ka;  // ee
+    acc[1] = acc[1] + fac * scra;       // zz
+    acc[2] = acc[2] + dnu;              // disz_nu
+    acc[3] = acc[3] + dmu;              // disz_mu
+    acc[4] = acc[4] + dnu / ka;         // dise_nu
+    acc[5] = acc[5] + dmu / ka;         // dise_mu
   }
-  __shared__ double red[6][256];
+  __shared__ double red[6][kDiagT / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = acc[q];
+  for (int q = 0; q < 6; ++q) {
+    const double v = wave_sum(acc[q]);
+    if (lane == 0) red[q][wave] = v;
+  }
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-#pragma unroll
-      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    double v = red[q][0];
+    for (int w = 1; w < kDiagT / 64; ++w) v = v + red[q][w];
+    d.gpart[(size_t)c * 6 + q] = v;
   }
-  if (threadIdx.x < 6) d.gpart[(size_t)c * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void glob_reduce_k(DiagArgs d) {
-  const int q = threadIdx.x;
-  if (q >= 6) return;
+// One wavefront per quantity, over the columns in a fixed order.
+__global__ void __launch_bounds__(6 * 64) glob_sum_k(DiagArgs d) {
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0.0;
-  for (int c = 0; c <= d.rc.kx; ++c) s = s + d.gpart[(size_t)c * 6 + q];
-  d.gout[q] = s;
+  for (int c = lane; c <= d.rc.kx; c += 64) s = s + d.gpart[(size_t)c * 6 + q];
+  s = wave_sum(s);
+  if (lane == 0) d.gout[q] = s;
 }
 
 __global__ void shell_index_k(Phys ph, int* ib) {
@@ -1652,15 +1799,33 @@ hipError_t launch_ens_tick(int* step, hipStream_t s) {
 
 hipError_t launch_spectrum(const DiagArgs& d, int which, hipStream_t s) {
   const int nbin = d.ph.n / 2;
-  shell_part_k<<<d.rc.kc(), 256, 0, s>>>(d, which);
-  shell_reduce_k<<<(nbin + 255) / 256, 256, 0, s>>>(d, which);
+  int nbl = 1;  // the most shells of one column
+  for (int c = 0; c <= d.rc.kx; ++c) {
+    int lo, hi;
+    column_shells(d.ph, d.rc, c, lo, hi);
+    nbl = hi - lo + 1 > nbl ? hi - lo + 1 : nbl;
+  }
+  // LDS: two sums per shell of a column, the chunk's run sums and keys, and
+  // one slot per wave
+  const size_t lds = sizeof(double) * (2 * (size_t)nbl + 2 * kDiagT) +
+                     sizeof(int) * (kDiagT + kDiagT / 64);
+  static bool attr = false;  // once per process: more than 64 KB of LDS at N = 8192
+  if (!attr && lds > 48 * 1024) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&shell_bin_k),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  shell_bin_k<<<d.rc.kc(), kDiagT, lds, s>>>(d, which, nbl);
+  shell_sum_k<<<(nbin * 64 + kDiagT - 1) / kDiagT, kDiagT, 0, s>>>(d, which);
   if (which == 1) suffix_k<<<(nbin + 255) / 256, 256, 0, s>>>(d);
   return hipGetLastError();
 }
 
 hipError_t launch_glob(const DiagArgs& d, hipStream_t s) {
-  glob_part_k<<<d.rc.kc(), 256, 0, s>>>(d);
-  glob_reduce_k<<<1, 64, 0, s>>>(d);
+  glob_col_k<<<d.rc.kc(), kDiagT, 0, s>>>(d);
+  glob_sum_k<<<1, 6 * 64, 0, s>>>(d);
   return hipGetLastError();
 }
 

@@ -209,6 +209,27 @@ int rect_tables(gt_ctx* c, const Rect& r, RectTables** out) {
   HIPCHK(hipMalloc(&t.modes, sizeof(ForcedMode) * fm.size()));
   HIPCHK(hipMemcpy(t.modes, fm.data(), sizeof(ForcedMode) * fm.size(),
                    hipMemcpyHostToDevice));
+  // The same updates by column for the fused step boundary, each column's in
+  // table order (a mode, then its partner).
+  std::vector<std::vector<gt::FEntry>> bycol(r.kc());
+  for (int k = 0; k < c->nf; ++k) {
+    const int col = c->ikxf[k] - 1;
+    const ForcedMode& f = fm[k];
+    bycol[col].push_back(gt::FEntry{f.idx - col * t.kyp, k, 0, f.ff1, f.ff2});
+    if (f.pidx >= 0) bycol[col].push_back(gt::FEntry{f.pidx - col * t.kyp, k, 1, f.ff1, f.ff2});
+  }
+  std::vector<int> fcs(r.kc() + 1, 0);
+  std::vector<gt::FEntry> fce;
+  for (int col = 0; col < r.kc(); ++col) {
+    fcs[col] = (int)fce.size();
+    fce.insert(fce.end(), bycol[col].begin(), bycol[col].end());
+  }
+  fcs[r.kc()] = (int)fce.size();
+  if (fce.empty()) fce.push_back(gt::FEntry{-1, 0, 0, 0.0, 0.0});
+  HIPCHK(hipMalloc(&t.fcs, sizeof(int) * fcs.size()));
+  HIPCHK(hipMemcpy(t.fcs, fcs.data(), sizeof(int) * fcs.size(), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&t.fce, sizeof(gt::FEntry) * fce.size()));
+  HIPCHK(hipMemcpy(t.fce, fce.data(), sizeof(gt::FEntry) * fce.size(), hipMemcpyHostToDevice));
   return GT_OK;
 }
 
@@ -324,7 +345,8 @@ gt::PassArgs pass_args(gt_ctx* c, Mirror& z, RectTables* t) {
 int c_pass(gt_ctx* c, int mode, const gt::PassArgs& a) {
   const int cls = mode == gt::C_START ? GT_K_YSTART
                 : (mode == gt::C_RK4_4 || mode == gt::C_RK2_2 || mode == gt::C_NLT)
-                    ? GT_K_YFINAL : GT_K_YSTAGE;
+                    ? GT_K_YFINAL
+                : (mode == gt::C_RK4_4S || mode == gt::C_RK2_2S) ? GT_K_YBOUND : GT_K_YSTAGE;
   Timer t(c, cls);
   HIPCHK(gt::launch_c_pass(c->logn, mode, a, c->stream));
   return GT_OK;
@@ -370,6 +392,8 @@ gt::DiagArgs diag_args(gt_ctx* c, const Mirror& z) {
   d.kyp = z.kyp;
   d.ph = c->ph;
   d.z = z.dcmp;
+  d.zcs = z.kyp;
+  d.zrs = 1;
   const size_t nbin = c->n / 2, kc = z.rect.kc();
   d.part1 = c->dg;
   d.part2 = d.part1 + (kc + 1) * nbin;
@@ -399,21 +423,81 @@ int same_rect(gt_ctx* c, Mirror& a, Mirror& b) {
 
 // src/forcing.f90 on a compact field: the phases of this call (host LCG)
 // go through a ring of pinned slots to the device.
-int forcing_on(gt_ctx* c, RectTables* t, double2* z, int32_t* seed) {
-  if (c->nf == 0) return GT_OK;
+// The next forcing draws (src/forcing.f90:16-24) into a free slot of the
+// phase ring, copied to the device on the stream; the slot is marked used
+// once its reader is launched (slot_read).
+int forcing_upload(gt_ctx* c, int32_t* seed, unsigned* slot_out) {
   const unsigned slot = c->ph_next++ % kRing;
   if (c->ph_ev_used[slot]) HIPCHK(hipEventSynchronize(c->ph_ev[slot]));
   double2* hp = c->ph_host + (size_t)slot * c->nf;
   double2* dp = c->ph_dev + (size_t)slot * c->nf;
   forcing_phases(c->nf, seed, hp);
   HIPCHK(hipMemcpyAsync(dp, hp, sizeof(double2) * c->nf, hipMemcpyHostToDevice, c->stream));
+  *slot_out = slot;
+  return GT_OK;
+}
+int slot_read(gt_ctx* c, unsigned slot) {
+  HIPCHK(hipEventRecord(c->ph_ev[slot], c->stream));
+  c->ph_ev_used[slot] = true;
+  return GT_OK;
+}
+int forcing_slot(gt_ctx* c, RectTables* t, double2* z, unsigned slot) {
+  double2* dp = c->ph_dev + (size_t)slot * c->nf;
   {
     Timer tt(c, GT_K_FORCING);
     HIPCHK(gt::launch_forcing(c->ph, c->nf, t->modes, dp, c->sdt * c->famp, z, c->stream));
   }
-  HIPCHK(hipEventRecord(c->ph_ev[slot], c->stream));
-  c->ph_ev_used[slot] = true;
+  return slot_read(c, slot);
+}
+int forcing_on(gt_ctx* c, RectTables* t, double2* z, int32_t* seed) {
+  if (c->nf == 0) return GT_OK;
+  unsigned slot;
+  CHK(forcing_upload(c, seed, &slot));
+  return forcing_slot(c, t, z, slot);
+}
+
+// ---- the deferred end of a step ----------------------------------------------
+
+int flush_pending(gt_ctx* c) {
+  Pending& p = c->pend;
+  if (p.kind == PEND_NONE) return GT_OK;
+  const Pending q = p;
+  p = Pending{};
+  CHK(c_pass(c, q.kind == PEND_RK4 ? gt::C_RK4_4 : gt::C_RK2_2, q.a));
+  if (q.forced) CHK(forcing_slot(c, q.t, q.a.z, q.slot));
   return GT_OK;
+}
+
+// Whether the deferred end can be fused into a step of the same kind on
+// the same arrays (state z, and for RK4 the accumulator of the deferred
+// step, which its last stage reads).
+static bool pending_fuses(const gt_ctx* c, int kind, const Mirror* m, const Mirror* mn) {
+  const Pending& p = c->pend;
+  return p.kind == kind && p.z == m && (kind == PEND_RK2 || p.znew == mn) &&
+         p.a.rc.kx == m->rect.kx && p.a.rc.p == m->rect.p && p.a.rc.q == m->rect.q &&
+         p.a.z == m->dcmp && (kind == PEND_RK2 || p.a.znew == mn->dcmp);
+}
+
+// First pass of a step: the fused boundary of the deferred step when it
+// fuses (its last stage, its forcing, this step's inverse part), else the
+// inverse part alone.
+static int step_start(gt_ctx* c, int kind, Mirror* m, Mirror* mn, const gt::PassArgs& a) {
+  if (pending_fuses(c, kind, m, mn)) {
+    const Pending q = c->pend;
+    c->pend = Pending{};
+    gt::PassArgs b = q.a;
+    if (q.forced) {
+      b.fcs = q.t->fcs;
+      b.fce = q.t->fce;
+      b.fph = c->ph_dev + (size_t)q.slot * c->nf;
+      b.famp = c->sdt * c->famp;
+    }
+    CHK(c_pass(c, kind == PEND_RK4 ? gt::C_RK4_4S : gt::C_RK2_2S, b));
+    if (q.forced) CHK(slot_read(c, q.slot));
+    return GT_OK;
+  }
+  CHK(flush_pending(c));
+  return c_pass(c, gt::C_START, a);
 }
 
 }  // namespace gtr
@@ -533,6 +617,7 @@ int gt_create(int32_t n, const double* p, gt_ctx** out) {
 
 int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
+  c->pend = Pending{};  // a deferred step end is dropped with the context
   if (c->stream) hipStreamSynchronize(c->stream);
   adj_free(c);
   for (auto& kv : c->present) {
@@ -543,6 +628,8 @@ int gt_destroy(gt_ctx* c) {
   for (auto& t : c->tab) {
     if (t.emk) hipFree(t.emk);
     if (t.modes) hipFree(t.modes);
+    if (t.fcs) hipFree(t.fcs);
+    if (t.fce) hipFree(t.fce);
   }
   if (c->tw) hipFree(c->tw);
   if (c->y1) hipFree(c->y1);
@@ -625,6 +712,7 @@ int gt_mem_info(size_t* fr, size_t* tot) {
 int gt_enter_data(gt_ctx* c, void* host, int64_t bytes, int32_t copyin) {
   GT_RANGE();
   if (!c || !host || bytes <= 0) return fail(GT_ERR_ARG, "bad enter_data arguments");
+  CHK(flush_pending(c));
   if (c->present.count(host)) return fail(GT_ERR_ARG, "array %p already present", host);
   Mirror m;
   m.host = host;
@@ -643,6 +731,7 @@ int gt_enter_data(gt_ctx* c, void* host, int64_t bytes, int32_t copyin) {
 
 int gt_map_data(gt_ctx* c, void* key, void* dev, int64_t bytes) {
   if (!c || !key || !dev || bytes <= 0) return fail(GT_ERR_ARG, "bad map_data arguments");
+  CHK(flush_pending(c));
   if (c->present.count(key)) return fail(GT_ERR_ARG, "key %p already present", key);
   Mirror m;
   m.host = nullptr;
@@ -666,6 +755,7 @@ static int drop(gt_ctx* c, const void* key) {
 
 int gt_unmap_data(gt_ctx* c, void* key) {
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   auto it = c->present.find(key);
   if (it == c->present.end()) return fail(GT_ERR_NOT_PRESENT, "key %p not present", key);
   if (it->second.owned) return fail(GT_ERR_ARG, "key %p was not mapped", key);
@@ -676,6 +766,7 @@ int gt_unmap_data(gt_ctx* c, void* key) {
 int gt_exit_data(gt_ctx* c, void* host, int32_t copyout) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   auto it = c->present.find(host);
   if (it == c->present.end()) return fail(GT_ERR_NOT_PRESENT, "array %p not present", host);
   Mirror& m = it->second;
@@ -690,6 +781,7 @@ int gt_exit_data(gt_ctx* c, void* host, int32_t copyout) {
 int gt_update_host(gt_ctx* c, void* host) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   auto it = c->present.find(host);
   if (it == c->present.end()) return fail(GT_ERR_NOT_PRESENT, "array %p not present", host);
   Mirror& m = it->second;
@@ -706,6 +798,7 @@ int gt_sync_ref(gt_ctx* c, void* key) { return gt_update_host(c, key); }
 int gt_update_device(gt_ctx* c, void* host) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   auto it = c->present.find(host);
   if (it == c->present.end()) return fail(GT_ERR_NOT_PRESENT, "array %p not present", host);
   Mirror& m = it->second;
@@ -720,6 +813,7 @@ int gt_update_device(gt_ctx* c, void* host) {
 int gt_copy(gt_ctx* c, void* dst, const void* src) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   Mirror *d, *s;
   CHK(lookup(c, dst, &d));
   CHK(lookup(c, src, &s));
@@ -746,6 +840,7 @@ int gt_copy(gt_ctx* c, void* dst, const void* src) {
 
 int gt_synchronize(gt_ctx* c) {
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   HIPCHK(hipStreamSynchronize(c->stream));
   return GT_OK;
 }
@@ -755,6 +850,7 @@ int gt_synchronize(gt_ctx* c) {
 int gt_fft_dir(gt_ctx* c, double* z, int32_t plan) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   if (plan != GT_PLAN_D2Z) return fail(GT_ERR_ARG, "fft_dir needs a D2Z plan (got %d)", plan);
   Mirror* m;
   CHK(lookup(c, z, &m));
@@ -768,6 +864,7 @@ int gt_fft_dir(gt_ctx* c, double* z, int32_t plan) {
 int gt_fft_inv(gt_ctx* c, double* z, int32_t plan) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   if (plan != GT_PLAN_Z2D) return fail(GT_ERR_ARG, "fft_inv needs a Z2D plan (got %d)", plan);
   Mirror* m;
   CHK(lookup(c, z, &m));
@@ -781,6 +878,7 @@ int gt_fft_inv(gt_ctx* c, double* z, int32_t plan) {
 int gt_produ(gt_ctx* c, const double* z, double* u, double* v) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   Mirror *mz, *mu, *mv;
   CHK(lookup(c, z, &mz));
   CHK(lookup(c, u, &mu));
@@ -798,6 +896,7 @@ int gt_produ(gt_ctx* c, const double* z, double* u, double* v) {
 int gt_trunc(gt_ctx* c, double* z) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   Mirror* m;
   CHK(lookup(c, z, &m));
   Timer t(c, GT_K_OTHER);
@@ -818,6 +917,7 @@ int gt_trunc(gt_ctx* c, double* z) {
 int gt_nlt(gt_ctx* c, double* z, double* u, double* v) {
   GT_RANGE();
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   Mirror *m, *mu, *mv;
   CHK(lookup(c, z, &m));
   CHK(lookup(c, u, &mu));
@@ -841,16 +941,19 @@ int gt_step(gt_ctx* c, double* z, double* znl, double* u, double* v) {
   CHK(lookup(c, znl, &m1));
   CHK(lookup(c, u, &mu));
   CHK(lookup(c, v, &mv));
+  if (!pending_fuses(c, PEND_RK2, m, nullptr)) CHK(flush_pending(c));
   CHK(ensure_cmp(c, *m));
   CHK(ensure_ybuf(c, m->rect));
   RectTables* t;
   CHK(rect_tables(c, m->rect, &t));
   gt::PassArgs a = pass_args(c, *m, t);
-  CHK(c_pass(c, gt::C_START, a));
+  CHK(step_start(c, PEND_RK2, m, nullptr, a));
   CHK(r_pass(c, a));
   CHK(c_pass(c, gt::C_RK2_1, a));
   CHK(r_pass(c, a));
-  CHK(c_pass(c, gt::C_RK2_2, a));
+  // the last stage (C_RK2_2) is deferred: the next step fuses it with its
+  // forcing and its first pass when nothing else comes between
+  c->pend = Pending{PEND_RK2, m, nullptr, t, a, false, 0};
   set_cmp_only(*m);
   return GT_OK;
 }
@@ -865,6 +968,7 @@ int gt_step4(gt_ctx* c, double* z, double* znl, double* znew, double* u, double*
   CHK(lookup(c, u, &mu));
   CHK(lookup(c, v, &mv));
   if (mn == m) return fail(GT_ERR_ARG, "step4: z and znew must differ");
+  if (!pending_fuses(c, PEND_RK4, m, mn)) CHK(flush_pending(c));
   CHK(ensure_cmp(c, *m));
   CHK(ensure_ybuf(c, m->rect));
   RectTables* t;
@@ -872,7 +976,7 @@ int gt_step4(gt_ctx* c, double* z, double* znl, double* znew, double* u, double*
   CHK(alloc_cmp(c, *mn, m->rect));
   gt::PassArgs a = pass_args(c, *m, t);
   a.znew = mn->dcmp;
-  CHK(c_pass(c, gt::C_START, a));
+  CHK(step_start(c, PEND_RK4, m, mn, a));
   CHK(r_pass(c, a));
   CHK(c_pass(c, gt::C_RK4_1, a));
   CHK(r_pass(c, a));
@@ -880,7 +984,9 @@ int gt_step4(gt_ctx* c, double* z, double* znl, double* znew, double* u, double*
   CHK(r_pass(c, a));
   CHK(c_pass(c, gt::C_RK4_3, a));
   CHK(r_pass(c, a));
-  CHK(c_pass(c, gt::C_RK4_4, a));
+  // the last stage (C_RK4_4) is deferred: the next step fuses it with its
+  // forcing and its first pass when nothing else comes between
+  c->pend = Pending{PEND_RK4, m, mn, t, a, false, 0};
   set_cmp_only(*m);
   set_cmp_only(*mn);
   return GT_OK;
@@ -892,6 +998,14 @@ int gt_forcing(gt_ctx* c, double* z, int32_t* seed) {
   Mirror* m;
   CHK(lookup(c, z, &m));
   if (c->nf == 0) return GT_OK;
+  Pending& p = c->pend;
+  if (p.kind != PEND_NONE && p.z == m && !p.forced) {
+    // after a deferred step end: draw now, apply with that end
+    CHK(forcing_upload(c, seed, &p.slot));
+    p.forced = true;
+    return GT_OK;
+  }
+  CHK(flush_pending(c));
   CHK(ensure_cmp(c, *m));
   RectTables* t;
   CHK(rect_tables(c, m->rect, &t));
@@ -905,6 +1019,7 @@ int gt_forcing(gt_ctx* c, double* z, int32_t* seed) {
 int gt_spectrum(gt_ctx* c, const double* z, double* sp, double* sz) {
   GT_RANGE();
   if (!c || !sp || !sz) return fail(GT_ERR_ARG, "null argument");
+  CHK(flush_pending(c));
   Mirror* m;
   CHK(lookup(c, z, &m));
   CHK(ensure_cmp(c, *m));
@@ -924,6 +1039,7 @@ int gt_spectrum(gt_ctx* c, const double* z, double* sp, double* sz) {
 int gt_fluxes(gt_ctx* c, const double* z, const double* znl, double* fle, double* flz) {
   GT_RANGE();
   if (!c || !fle || !flz) return fail(GT_ERR_ARG, "null argument");
+  CHK(flush_pending(c));
   Mirror *m, *mn;
   CHK(lookup(c, z, &m));
   CHK(lookup(c, znl, &mn));
@@ -947,6 +1063,7 @@ int gt_fluxes(gt_ctx* c, const double* z, const double* znl, double* fle, double
 int gt_glob(gt_ctx* c, const double* z, double* out) {
   GT_RANGE();
   if (!c || !out) return fail(GT_ERR_ARG, "null argument");
+  CHK(flush_pending(c));
   Mirror* m;
   CHK(lookup(c, z, &m));
   CHK(ensure_cmp(c, *m));
@@ -991,6 +1108,7 @@ int gt_timing_classes(gt_ctx* c, int32_t mask) {
 
 int gt_timing_read(gt_ctx* c, double* ms, int32_t* count) {
   if (!c) return fail(GT_ERR_ARG, "null context");
+  CHK(flush_pending(c));
   HIPCHK(hipStreamSynchronize(c->stream));
   for (int i = 0; i < GT_K_NCLASS; ++i) {
     if (ms) ms[i] = 0.0;

@@ -21,8 +21,8 @@ from .params import Params
 PLAN_D2Z = 1
 PLAN_Z2D = 2
 
-K_YSTART, K_YSTAGE, K_YFINAL, K_XPASS, K_FORCING, K_OTHER = range(6)
-K_NAMES = ["y_start", "y_stage", "y_final", "x_pass", "forcing", "other"]
+K_YSTART, K_YSTAGE, K_YFINAL, K_XPASS, K_FORCING, K_OTHER, K_YBOUND = range(7)
+K_NAMES = ["y_start", "y_stage", "y_final", "x_pass", "forcing", "other", "y_bound"]
 
 
 def _addr(a):
@@ -229,10 +229,10 @@ class Solver:
         check(self.L.gt_timing_enable(self.h, 1 if on else 0))
 
     def timing_read(self):
-        ms = np.zeros(6)
-        cnt = np.zeros(6, np.int32)
+        ms = np.zeros(len(K_NAMES))
+        cnt = np.zeros(len(K_NAMES), np.int32)
         check(self.L.gt_timing_read(self.h, ms.ctypes.data, cnt.ctypes.data))
-        return {K_NAMES[i]: (float(ms[i]), int(cnt[i])) for i in range(6)}
+        return {K_NAMES[i]: (float(ms[i]), int(cnt[i])) for i in range(len(K_NAMES))}
 
     # ---- the driver loop (src/main.f90:89-143) ---------------------------------
     def run(self, zphys, seed, npas, t=0.0, diag=None):

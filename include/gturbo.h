@@ -111,7 +111,15 @@ int gt_nlt(gt_ctx *ctx, double *z, double *u, double *v);
  * are work arrays. */
 int gt_step(gt_ctx *ctx, double *z, double *znl, double *u, double *v);
 /* src/rkstep.f90:82-131: Lawson RK4.  znew receives the stage accumulator
- * (src/rkstep.f90:96-114); znl, u, v are work arrays. */
+ * (src/rkstep.f90:96-114); znl, u, v are work arrays.
+ *
+ * gt_step and gt_step4 leave their last stage (the update of z) queued in
+ * the context: when the next call on the context is gt_forcing(z) and then
+ * the same step on the same arrays -- the driver's loop, src/main.f90:97-136
+ * -- that stage, the forcing and the next step's first pass run as one pass
+ * over the fields.  Any other entry point runs the queued work first, so
+ * every result a caller can observe is the same as with the three calls run
+ * one after the other. */
 int gt_step4(gt_ctx *ctx, double *z, double *znl, double *znew, double *u,
              double *v);
 /* src/forcing.f90:1-51: adds one draw of the ring forcing and advances the
@@ -140,7 +148,9 @@ int gt_shell_index(gt_ctx *ctx, int32_t *ib);
 #define GT_K_XPASS 3  /* x pass */
 #define GT_K_FORCING 4
 #define GT_K_OTHER 5
-#define GT_K_NCLASS 6
+#define GT_K_YBOUND 6 /* y pass: a step's last stage + its forcing + the next
+                         step's inverse part (see gt_step4) */
+#define GT_K_NCLASS 7
 int gt_timing_enable(gt_ctx *ctx, int32_t on);
 /* Which classes gt_timing_enable times: bit (1 << GT_K_*) set = timed
  * (default: all).  Each timed launch records two events on the stream;

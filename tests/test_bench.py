@@ -33,9 +33,12 @@ def test_committed_counters_match_the_byte_model():
     n = w.params.n
     m = n // 3  # the 2/3 rectangle: kx, ky in [-N/3, N/3]
     cb = bench.compact_bytes(n, (m, m, m), 4)
+    for k in ("x_pass", "y_stage"):
+        assert bench.read_traffic(k) is not None, k
     for k, v in cb.items():
         t = bench.read_traffic(k)
-        assert t is not None, k
+        if t is None:
+            continue
         assert abs(t / v - 1.0) < (0.05 if k == "x_pass" else 0.005), (k, t, v)
     # one RK4 step: y_start + 3 y_stage + y_final + 4 x passes
     per_step = cb["y_start"] + 3 * cb["y_stage"] + cb["y_final"] + 4 * cb["x_pass"]
@@ -78,7 +81,17 @@ def test_bench_line_has_the_contract_keys():
     assert roof["bound"] == "hbm" and roof["unit"] == "GB/s" and roof["peak"] == 8000.0
     assert abs(roof["frac"] - roof["achieved"] / roof["peak"]) < 1e-12
     assert roof["traffic"] is None  # the counter summary is of the 4096^2 workload only
+    assert roof["frac_measured"] is None and roof["achieved"] <= roof["peak"]
+    assert roof["model_frac"] == pytest.approx(roof["model_achieved"] / roof["peak"])
+    for k, v in d["kernels"].items():
+        assert v.get("achieved_GBs", 0.0) <= 8000.0, k
     cb = d["cpu_baseline"]
     assert cb["value"] > 0 and cb["kind"] == "port" and cb["cores"] >= 1 and cb["sample"]
+    assert cb["one_core"]["cores"] == 1 and cb["one_core"]["value"] > 0
+    assert cb["cpu_model"] and cb["nproc"] >= 1
+    e = cb["euler256"]
+    assert e["all_cores"]["value"] > 0 and e["one_core"]["cores"] == 1
     for leg in ("rk2", "ensemble", "adjoint"):
         assert d[leg]["finite"] is True, leg
+    for leg in ("euler256", "sqg1024"):
+        assert d["configs"][leg]["finite"] is True, leg

@@ -181,3 +181,36 @@ def test_ensemble_workload_every_member_matches_solo():
         assert np.array_equal(z, zs), (m, rel_l2(z, zs))
     print("worst rel L2", worst, flush=True)
     ens.close()
+
+
+@pytest.mark.parametrize("cfg", [None, [256, 512, 0, 0, 3072, 10]])
+def test_workload_members_of_every_size_match_oracle(oracle_mod, cfg):
+    """Members of the batched-ensemble workload at 256^2, 512^2 and 1024^2
+    (alpha 1/2, 1 and 2, their own ring, nu, dt and seeds) advanced together
+    for 20 steps, with the default schedule and with the schedule the
+    autotuner picked for the whole workload on MI355X: each member within
+    1e-10 relative L2 of the oracle, forcing seeds equal."""
+    members = ensemble_members()
+    pick = [0, 31, 33, 34, 56, 57, 58]  # 256/0.5, 256/1, 512/0.5, 512/1, 1024/2, 1024/0.5, 1024/1
+    ws = [members[m] for m in pick]
+    assert sorted({w.params.n for w in ws}) == [256, 512, 1024]
+    assert sorted({w.params.alpv for w in ws}) == [0.5, 1.0, 2.0]
+    ps = [w.params for w in ws]
+    fields = [ic.physical_field(w.params, w.ic) for w in ws]
+    seeds = [w.forcing_seed for w in ws]
+    nsteps = 20
+    out, sd = _batched(ps, fields, seeds, nsteps, cfg)
+    for k, p in enumerate(ps):
+        o = oracle_mod.Oracle(p)
+        z = np.zeros((p.n, p.n + 2))
+        z[:, : p.n] = fields[k][:, : p.n]
+        o.fft_dir(z)
+        o.trunc(z)
+        seed = seeds[k]
+        for _ in range(nsteps):
+            o.step4(z)
+            seed = o.forcing(z, seed)
+        err = rel_l2(out[k], z)
+        print(f"member {pick[k]} N={p.n} alpha={p.alpv}: rel L2 {err:.2e}", flush=True)
+        assert sd[k] == seed
+        assert err < 1e-10, (pick[k], err)

@@ -84,6 +84,10 @@ def test_fortran_driver_matches_oracle(oracle_mod, tmp_path, rko):
     for d, (ipas, sp, sz, fle, flz, g) in enumerate(diag):
         assert glob[d, 0] == pytest.approx(ipas * p.dt)
         assert np.allclose(glob[d, [2, 1, 3, 4, 6, 7]], g, rtol=1e-9, atol=0)
+        # inpz, inpe of the library's iniforcing (src/iniforcing.f90:27-28,
+        # src/diagnostics.f90:166)
+        assert glob[d, 5] == pytest.approx(o.inpz, rel=1e-14)
+        assert glob[d, 8] == pytest.approx(o.inpe, rel=1e-14)
         assert rel_l2(spec[d, :, 1], sp) < 1e-9 and rel_l2(spec[d, :, 2], sz) < 1e-9
         assert np.abs(flux[d, :, 2] - flz).max() <= 1e-9 * np.abs(flz).max()
         assert np.abs(flux[d, :, 1] - fle).max() <= 1e-9 * np.abs(fle).max()

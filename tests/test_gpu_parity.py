@@ -18,14 +18,19 @@ def _pair(oracle_mod, p):
     return Solver(p), oracle_mod.Oracle(p)
 
 
+def _params_fft(n):
+    # at 8192 the default ring (N/16 +- 1) holds more than NFORC modes
+    return params_for(n, ring=(63.0, 65.0) if n >= 8192 else None)
+
+
 def _present(s, *arrays):
     for a in arrays:
         s.enter_data(a)
 
 
-@pytest.mark.parametrize("n", SIZES + [1024, 4096])
+@pytest.mark.parametrize("n", SIZES + [1024, 4096, 8192])
 def test_fft_dir_matches_oracle(oracle_mod, n):
-    p = params_for(n)
+    p = _params_fft(n)
     s, o = _pair(oracle_mod, p)
     z = random_physical(n, 10 + n)
     zg = z.copy()
@@ -36,9 +41,9 @@ def test_fft_dir_matches_oracle(oracle_mod, n):
     assert rel_l2(zg, z) < 1e-14
 
 
-@pytest.mark.parametrize("n", SIZES + [1024, 4096])
+@pytest.mark.parametrize("n", SIZES + [1024, 4096, 8192])
 def test_fft_inv_matches_oracle(oracle_mod, n):
-    p = params_for(n)
+    p = _params_fft(n)
     s, o = _pair(oracle_mod, p)
     z = random_spectral(n, 20 + n, truncated=False)
     zg = z.copy()
@@ -175,6 +180,46 @@ def test_forcing_bit_exact(oracle_mod, n, dt):
         print("mismatches (ky_index, kx):", [tuple(x) for x in cb],
               [(zc[tuple(x)], gc[tuple(x)]) for x in cb])
     assert len(bad) == 0
+
+
+@pytest.mark.parametrize("n,ring,alpv,famp", [
+    (256, (15.0, 17.0), 2.0, 1.0), (512, (31.0, 33.0), 1.0, 0.7),
+    (1024, (31.0, 33.0), 1.0, 1.0), (4096, (63.0, 65.0), 2.0, 1.0),
+    (8192, (63.0, 65.0), 0.5, 1.3)])
+def test_iniforcing_tables_match_oracle(oracle_mod, n, ring, alpv, famp):
+    """The library's src/iniforcing.f90 set-up: the forced modes in the
+    reference's order (ikxf, ikyf, ff), nf, and the injection sums inpz =
+    sum famp^2 and inpe = sum famp^2/k^alpha (iniforcing.f90:27-28), which
+    feed files/global.NNN and the Zb/Eb line."""
+    p = params_for(n, ring=ring, alpv=alpv, famp=famp)
+    s, o = _pair(oracle_mod, p)
+    assert s.nf == o.nf and s.nf > 0
+    ikx, iky, ff = s.forcing_modes()
+    oikx, oiky, off = o.forcing_modes()
+    assert np.array_equal(ikx, oikx) and np.array_equal(iky, oiky)
+    assert np.array_equal(ff, off)
+    assert s.inpz == o.inpz and s.inpe == o.inpe  # same sums in the same order
+    s.close()
+
+
+def test_iniforcing_overflow_is_nforc_error():
+    """More forced modes than NFORC = 2048 (src/modules.f90:41) stops the
+    reference with 'NFORC too small !' (src/iniforcing.f90:22); gt_create
+    returns GT_ERR_NFORC with that message and no context."""
+    import ctypes as C
+    from gturbo_amd import _lib
+    L = _lib.load()
+    p = params_for(4096, ring=(200.0, 210.0))  # about 6,400 modes in the ring
+    arr = np.ascontiguousarray(p.as_array(), np.float64)
+    h = C.c_void_p(1)
+    rc = L.gt_create(4096, arr.ctypes.data, C.byref(h))
+    assert rc == 4  # GT_ERR_NFORC
+    assert L.gt_last_error().decode() == "NFORC too small !"
+    assert not h.value
+    p = params_for(4096, ring=(63.0, 65.0))  # the library is usable afterwards
+    arr = np.ascontiguousarray(p.as_array(), np.float64)
+    assert L.gt_create(4096, arr.ctypes.data, C.byref(h)) == 0
+    assert L.gt_destroy(h) == 0
 
 
 @pytest.mark.parametrize("n", [16, 256, 4096])
@@ -399,3 +444,75 @@ def test_sqg1024_workload_with_diagnostics_matches_oracle(oracle_mod):
         assert np.abs(fle - ofle).max() <= 1e-9 * np.abs(ofle).max()
         assert np.abs(flz - oflz).max() <= 1e-9 * np.abs(oflz).max()
         assert np.allclose(g, row[4 * nb:], rtol=1e-9, atol=0)
+
+
+@pytest.mark.parametrize("n,rko", [(64, 4), (256, 4), (256, 2), (4096, 4)])
+def test_step_boundary_sequences_match_oracle(oracle_mod, n, rko):
+    """gt_step/gt_step4 leave their last stage queued; forcing(z) and the
+    next step on the same arrays fuse it with the next step's first pass,
+    anything else runs it first.  Two states of one context through the
+    sequences a caller can make -- step, forcing, step (fused), step without
+    forcing between (fused, unforced), a step on the other state (the queued
+    end runs first), forcing of the other state, diagnostics and a host copy
+    in between -- against the oracle doing the same operations."""
+    p = params_for(n, alpv=2.0 if n == 4096 else 1.0, rko=rko)
+    s, o = _pair(oracle_mod, p)
+    A = random_spectral(n, 301)
+    B = random_spectral(n, 302)
+    Ag, Bg = A.copy(), B.copy()
+    wa = [s.field() for _ in range(4)]
+    wb = [s.field() for _ in range(4)]
+    _present(s, Ag, Bg, *wa, *wb)
+
+    def gstep(z, w):
+        if rko == 4:
+            s.step4(z, *w)
+        else:
+            s.step(z, w[0], w[2], w[3])
+
+    def ostep(z):
+        (o.step4 if rko == 4 else o.step)(z)
+
+    sa = sb = 481929
+    oa = ob = 481929
+    # A: step, forcing, step (fused), forcing, step, step (fused without forcing)
+    for forced in (True, True, False):
+        gstep(Ag, wa)
+        ostep(A)
+        if forced:
+            sa = s.forcing(Ag, sa)
+            oa = o.forcing(A, oa)
+    gstep(Ag, wa)
+    ostep(A)
+    # B between A's step and A's forcing: A's queued end runs first
+    gstep(Bg, wb)
+    ostep(B)
+    sa = s.forcing(Ag, sa)
+    oa = o.forcing(A, oa)
+    sb = s.forcing(Bg, sb)
+    ob = o.forcing(B, ob)
+    # diagnostics between a step and its forcing, as the driver does
+    gstep(Ag, wa)
+    ostep(A)
+    g = s.glob(Ag)
+    assert np.allclose(g, o.glob(A), rtol=1e-12, atol=0)
+    sa = s.forcing(Ag, sa)
+    oa = o.forcing(A, oa)
+    gstep(Ag, wa)  # fused with the forcing above? no: the glob ran the end first
+    ostep(A)
+    s.update_host(Ag)
+    sa = s.forcing(Ag, sa)  # after a host copy: queued end already run
+    oa = o.forcing(A, oa)
+    gstep(Bg, wb)
+    ostep(B)
+    sb = s.forcing(Bg, sb)
+    ob = o.forcing(B, ob)
+    gstep(Bg, wb)  # fused with B's forcing
+    ostep(B)
+    s.update_host(Ag)
+    s.update_host(Bg)
+    assert (sa, sb) == (oa, ob)
+    ea, eb = rel_l2(Ag, A), rel_l2(Bg, B)
+    print(f"n={n} rk{rko}: A {ea:.2e} B {eb:.2e}", flush=True)
+    assert ea < 1e-12 and eb < 1e-12
+    s.close()
