@@ -243,6 +243,18 @@ def ensemble_bench(steps):
     ens.synchronize()
     tb = time.perf_counter() - t0
     finite = all(bool(np.all(np.isfinite(ens.get_field(m, E.SPECTRAL)))) for m in range(len(wl)))
+    # the same schedule on persistent grids (workgroups looping over the work
+    # items, 1/2/4 waves per SIMD) against the tuned one
+    persistent = {}
+    for wps in (1, 2, 4):
+        ens.schedule = [cfg[0], cfg[1], wps] + list(cfg[3:])
+        ens.run(5)
+        ens.synchronize()
+        t1 = time.perf_counter()
+        ens.run(max(steps // 5, 10))
+        ens.synchronize()
+        persistent[f"{wps}_waves_per_simd_ms_per_step"] = \
+            (time.perf_counter() - t1) / max(steps // 5, 10) * 1e3
     ens.close()
     tp = 0.0
     for w, f in zip(wl, fields):
@@ -274,6 +286,7 @@ def ensemble_bench(steps):
                         " RK4, alpha 1/2,1,2, forced, dt=0.25/N",
             "members": len(wl), "steps": steps, "schedule": cfg,
             "batched_ms_per_step": tb / steps * 1e3,
+            "persistent_grid": persistent,
             "plain_loop_ms_per_step": tp / steps * 1e3,
             "speedup": tp / tb,
             "batched_grid_pt_steps_per_s": gp * steps / tb,

@@ -114,20 +114,27 @@ __device__ __forceinline__ void line_map(int tid, int& line, int& j) {
 // y pass
 // ---------------------------------------------------------------------------
 
-// The forcing of compact row `row` of column c (fused step boundary): the
-// same operations as forcing_k's force_one on that element, in the order of
-// the forced-mode table.
-__device__ __forceinline__ void force_row(const PassArgs& a, int c, int row, double2& z) {
+// The forcing of column c (fused step boundary) on the thread's values
+// s[r] of rows p = j + (N/8) r: the same operations as forcing_k's
+// force_one on each forced element, in the order of the forced-mode table.
+template <int LOGN, int TR>
+__device__ __forceinline__ void force_column(const PassArgs& a, int c, bool colok, int j,
+                                             double2* s) {
 #pragma clang fp contract(off)
-  if (a.fcs == nullptr) return;
+  constexpr int N = 1 << LOGN, TPL = N / kE;
+  if (a.fcs == nullptr || !colok) return;
   const int e1 = a.fcs[c + 1];
   for (int e = a.fcs[c]; e < e1; ++e) {
     const FEntry f = a.fce[e];
-    if (f.row != row) continue;
     const double2 cs = a.fph[f.k];
     const double a1 = a.famp * f.ff1 * cs.x, a2 = a.famp * f.ff2 * cs.y;
-    z.x = z.x + a1;
-    z.y = f.conj ? z.y - a2 : z.y + a2;
+#pragma unroll
+    for (int r = 0; r < kE; ++r) {
+      if (live_r<TR>(r) && row_of(j + TPL * r, a.rc, N) == f.row) {
+        s[r].x = s[r].x + a1;
+        s[r].y = f.conj ? s[r].y - a2 : s[r].y + a2;
+      }
+    }
   }
 }
 
@@ -245,21 +252,17 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
           if (mean) zz = make_double2(0.0, 0.0);
           a.z[q] = zz;
         } else if constexpr (MODE == C_RK4_4S) {
-          // rkstep.f90:123,127, then forcing.f90:16-33 on this mode
+          // rkstep.f90:123,127; forced and stored below
           const double2 zn = a.znew[q];
           double2 zz = make_double2(zn.x + ph.dt6 * k.x, zn.y + ph.dt6 * k.y);
           if (mean) zz = make_double2(0.0, 0.0);
-          force_row(a, c, row, zz);
-          a.z[q] = zz;
-          s[r] = zz;  // the next step's stage input
+          s[r] = zz;
         } else if constexpr (MODE == C_RK2_2S) {
-          // rkstep.f90:54/64 and :72, then forcing.f90:16-33 on this mode
+          // rkstep.f90:54/64 and :72; forced and stored below
           const double E = a.emk[q];
           const double2 zz = a.z[q];
           double2 zo = make_double2(E * (E * zz.x + ph.dt * k.x), E * (E * zz.y + ph.dt * k.y));
           if (mean) zo = make_double2(0.0, 0.0);
-          force_row(a, c, row, zo);
-          a.z[q] = zo;
           s[r] = zo;
         } else if constexpr (MODE == C_RK2_1) {
           // rkstep.f90:24/34 and :42
@@ -275,6 +278,16 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
           if (mean) zo = make_double2(0.0, 0.0);
           a.z[q] = zo;
         }
+      }
+    }
+    if constexpr (MODE == C_RK4_4S || MODE == C_RK2_2S) {
+      // the step's forcing on this column (forcing.f90:16-33), then the
+      // forced state stored: it is also the next step's stage input
+      force_column<LOGN, TR>(a, c, colok, je, s);
+#pragma unroll
+      for (int r = 0; r < kE; ++r) {
+        const int row = row_of(je + TPL * r, rc, N);
+        if (live_r<TR>(r) && colok && row >= 0) a.z[cbase + (unsigned)row] = s[r];
       }
     }
   } else {
@@ -1569,17 +1582,43 @@ __global__ void __launch_bounds__(kDiagT) shell_sum_k(DiagArgs d, int which) {
   }
 }
 
-// flz(i) = sum(flzm(i:NBIN)) in the reference's order (diagnostics.f90:104-109)
-__global__ void suffix_k(DiagArgs d) {
+// flz(i) = sum(flzm(i:NBIN)) in the reference's order (diagnostics.f90:104-109):
+// each thread sums its own suffix in increasing q, from LDS (workgroup g
+// stages flzm and flem from its first i on).
+__global__ void __launch_bounds__(kDiagT) suffix_k(DiagArgs d) {
+  extern __shared__ double ssm[];
   const int nbin = d.ph.n / 2;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nbin) return;
+  const int i0 = blockIdx.x * kDiagT, len = nbin - i0;
   const double* flzm = d.part1 + (size_t)(d.rc.kx + 1) * nbin;
   const double* flem = d.part2 + (size_t)(d.rc.kx + 1) * nbin;
+  double* lz = ssm;
+  double* le = ssm + len;
+  for (int t = threadIdx.x; t < len; t += kDiagT) {
+    lz[t] = flzm[i0 + t];
+    le[t] = flem[i0 + t];
+  }
+  __syncthreads();
+  const int i = i0 + (int)threadIdx.x;
+  if (i >= nbin) return;
   double sz = 0.0, se = 0.0;
-  for (int q = i; q < nbin; ++q) {
-    sz = sz + flzm[q];
-    se = se + flem[q];
+  int q = i - i0;
+  // sixteen terms loaded together, then added one after the other in q order
+  for (; q + 16 <= len; q += 16) {
+    double tz[16], te[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      tz[t] = lz[q + t];
+      te[t] = le[q + t];
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      sz = sz + tz[t];
+      se = se + te[t];
+    }
+  }
+  for (; q < len; ++q) {
+    sz = sz + lz[q];
+    se = se + le[q];
   }
   d.out1[i] = se;  // fle
   d.out2[i] = sz;  // flz
@@ -1819,7 +1858,18 @@ hipError_t launch_spectrum(const DiagArgs& d, int which, hipStream_t s) {
   }
   shell_bin_k<<<d.rc.kc(), kDiagT, lds, s>>>(d, which, nbl);
   shell_sum_k<<<(nbin * 64 + kDiagT - 1) / kDiagT, kDiagT, 0, s>>>(d, which);
-  if (which == 1) suffix_k<<<(nbin + 255) / 256, 256, 0, s>>>(d);
+  if (which == 1) {
+    const size_t slds = 2 * sizeof(double) * (size_t)nbin;  // the first workgroup's
+    static bool sattr = false;
+    if (!sattr && slds > 48 * 1024) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&suffix_k),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)slds);
+      if (e != hipSuccess) return e;
+      sattr = true;
+    }
+    suffix_k<<<(nbin + kDiagT - 1) / kDiagT, kDiagT, slds, s>>>(d);
+  }
   return hipGetLastError();
 }
 
