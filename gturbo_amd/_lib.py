@@ -45,6 +45,7 @@ SIGNATURES = {
     "gt_spectrum": (C.c_int, [_P, _P, _P, _P]),
     "gt_fluxes": (C.c_int, [_P, _P, _P, _P, _P]),
     "gt_glob": (C.c_int, [_P, _P, _P]),
+    "gt_glob_noalloc": (C.c_int, [_P, _P, _P]),
     "gt_shell_index": (C.c_int, [_P, _P]),
     "gt_timing_enable": (C.c_int, [_P, _I32]),
     "gt_timing_classes": (C.c_int, [_P, _I32]),

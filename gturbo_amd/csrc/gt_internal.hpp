@@ -229,6 +229,10 @@ hipError_t launch_forcing(const Phys& ph, int nf, const ForcedMode* modes,
                           hipStream_t s);
 hipError_t launch_spectrum(const DiagArgs& d, int which, hipStream_t s);
 hipError_t launch_glob(const DiagArgs& d, hipStream_t s);
+// Device address of the static partial-sum array of the allocation-free glob
+// (gt_glob_noalloc) on the current device, resolved once per device, and its
+// size in doubles.
+hipError_t glob_scratch(double** ptr, size_t* count);
 hipError_t launch_shell_index(const Phys& ph, int* ib, hipStream_t s);
 // Batched ensemble passes: items are (member << 16 | tile or row group),
 // grid workgroups loop over them; sched is the radix order (fft_reg SCHED).

@@ -119,6 +119,10 @@ struct gt_ctx {
   // diagnostics
   double* dg = nullptr;
   size_t dg_cap = 0;
+  // device address of the static partial sums of gt_glob_noalloc on this
+  // context's device, and their count in doubles
+  double* glob_scratch = nullptr;
+  size_t glob_scratch_n = 0;
 
   // timing
   bool timing = false;

@@ -18,6 +18,8 @@
 // pass writes whole blocks (a 128-byte line at N = 4096), the x pass reads a
 // row's pieces of consecutive tiles from one DRAM page, and its workgroup
 // order keeps neighbouring rows on one XCD so that they share L2 lines.
+#include <mutex>
+
 #include "fft_core.hpp"
 #include "gt_internal.hpp"
 
@@ -1877,6 +1879,33 @@ hipError_t launch_glob(const DiagArgs& d, hipStream_t s) {
   glob_col_k<<<d.rc.kc(), kDiagT, 0, s>>>(d);
   glob_sum_k<<<1, 6 * 64, 0, s>>>(d);
   return hipGetLastError();
+}
+
+// Partial sums of the allocation-free glob: six per kx column (at most
+// N/2 + 1 = 4097 columns, N <= 8192) and the six totals, 196,704 bytes per
+// GPU for the life of the process.  Host code never uses the array's name
+// as an address: glob_scratch resolves the device address once per device.
+constexpr size_t kGlobScratch = 6 * (8192 / 2 + 1) + 6;
+static __device__ __align__(256) double scratch[kGlobScratch];
+
+hipError_t glob_scratch(double** ptr, size_t* count) {
+  constexpr int kMaxDev = 64;
+  static std::mutex mu;
+  static double* addr[kMaxDev] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDev) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (addr[dev] == nullptr) {
+    void* a = nullptr;
+    e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = static_cast<double*>(a);
+  }
+  *ptr = addr[dev];
+  *count = kGlobScratch;
+  return hipSuccess;
 }
 
 hipError_t launch_shell_index(const Phys& ph, int* ib, hipStream_t s) {

@@ -607,6 +607,8 @@ int gt_create(int32_t n, const double* p, gt_ctx** out) {
   CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   CREATE_CHK(upload_twiddles(c));
   CREATE_HIP(hipMalloc(&c->flag, sizeof(int)));
+  // the allocation-free glob's partial sums on this device (resolved once)
+  CREATE_HIP(gt::glob_scratch(&c->glob_scratch, &c->glob_scratch_n));
   const int nfa = c->nf > 0 ? c->nf : 1;
   CREATE_HIP(hipHostMalloc(&c->ph_host, sizeof(double2) * nfa * kRing, hipHostMallocDefault));
   CREATE_HIP(hipMalloc(&c->ph_dev, sizeof(double2) * nfa * kRing));
@@ -1075,6 +1077,39 @@ int gt_glob(gt_ctx* c, const double* z, double* out) {
   }
   HIPCHK(hipMemcpyAsync(out, d.gout, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  return GT_OK;
+}
+
+int gt_glob_noalloc(gt_ctx* c, const double* z, double* out_dev) {
+  GT_RANGE();
+  if (!c || !out_dev) return fail(GT_ERR_ARG, "null argument");
+  Mirror* m;
+  CHK(lookup(c, z, &m));
+  gt::DiagArgs d{};
+  d.ph = c->ph;
+  if (m->cmp_valid) {  // the compact copy, as gt_glob reads it
+    d.rc = m->rect;
+    d.kyp = m->kyp;
+    d.z = m->dcmp;
+    d.zcs = m->kyp;
+    d.zrs = 1;
+  } else if (m->ref_valid) {  // the reference layout: the whole half-spectrum
+    d.rc = c->rfull;
+    d.kyp = 0;
+    d.z = reinterpret_cast<const double2*>(m->dref);
+    d.zcs = 1;
+    d.zrs = c->nh;
+  } else {
+    return fail(GT_ERR_ARG, "array %p has no valid device data", z);
+  }
+  const size_t need = 6 * (size_t)d.rc.kc() + 6;
+  if (c->glob_scratch == nullptr || need > c->glob_scratch_n)
+    return fail(GT_ERR_ARG, "gt_glob_noalloc: %zu partial sums exceed the static array of %zu",
+                need, c->glob_scratch_n);
+  // every entry the launch reads it writes first: nothing to clear
+  d.gpart = c->glob_scratch;
+  d.gout = out_dev;
+  HIPCHK(gt::launch_glob(d, c->stream));
   return GT_OK;
 }
 

@@ -135,6 +135,17 @@ int gt_fluxes(gt_ctx *ctx, const double *z, const double *znl, double *fle,
               double *flz);
 /* glob (:125-179): out = ee zz disz_nu disz_mu dise_nu dise_mu. */
 int gt_glob(gt_ctx *ctx, const double *z, double *out);
+/* No counterpart in the reference: gt_glob for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same six sums, written to out_dev
+ * (6 doubles of device memory) asynchronously on the context's stream
+ * (gt_synchronize waits for them); nothing is allocated, copied to the host
+ * or synchronised.  The partial sums live in a static device array of
+ * 6*(N/2+1)+6 doubles for N <= 8192 (196,704 bytes per GPU, held until the
+ * process ends), shared by every context on that GPU: calls that overlap on
+ * one GPU race.  z must be present; its current device copy (compact or
+ * reference layout) is read as it is. */
+int gt_glob_noalloc(gt_ctx *ctx, const double *z, double *out_dev);
 /* Shell index int(k/dk)+1 of every half-spectrum mode as the device computes
  * it (diagnostics.f90:31-34), in Fortran order; 0 for the mean mode. */
 int gt_shell_index(gt_ctx *ctx, int32_t *ib);
