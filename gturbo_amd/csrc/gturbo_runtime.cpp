@@ -1081,8 +1081,9 @@ int gt_glob(gt_ctx* c, const double* z, double* out) {
 }
 
 int gt_glob_noalloc(gt_ctx* c, const double* z, double* out_dev) {
-  GT_RANGE();
+  // no profiler range here: nothing but kernel launches runs on this path
   if (!c || !out_dev) return fail(GT_ERR_ARG, "null argument");
+  CHK(flush_pending(c));  // a step's deferred end (launches only)
   Mirror* m;
   CHK(lookup(c, z, &m));
   gt::DiagArgs d{};

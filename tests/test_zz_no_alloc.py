@@ -2,8 +2,9 @@
 ABI: the same six sums (src/diagnostics.f90:125-179), written to device
 memory with the partial sums in the library's static array.  Two calls in a
 row, the second on a smaller grid (another context on the same GPU, so the
-array holds the first call's larger contents), then the first again; and a
-field whose only current device copy is in the reference layout."""
+array holds the first call's larger contents), then the first again, then
+right after a step; and a field whose only current device copy is in the
+reference layout."""
 import ctypes as C
 import os
 
@@ -60,6 +61,13 @@ def test_glob_noalloc_matches_glob():
             runs.append((s, z, want))
         s, z, want = runs[0]  # the larger grid again, after the smaller one
         assert np.array_equal(_glob_noalloc(s, z, R, dev), want)
+        # right after a step, whose last stage the context still holds back
+        work = [s.field() for _ in range(4)]
+        for a in work:
+            s.enter_data(a, copyin=False)
+        s.step4(z, *work)
+        got = _glob_noalloc(s, z, R, dev)
+        assert np.array_equal(got, s.glob(z)), "glob_noalloc saw the state before the step's end"
         # only the reference layout current: the whole half-spectrum is read
         n = 512
         s = Solver(params_for(n, alpv=0.5))
