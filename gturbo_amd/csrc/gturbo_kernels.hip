@@ -11,7 +11,8 @@
 //  * x pass (r_pass): one workgroup per RP y rows.  Complex-to-real x
 //    transforms of q+iv and q+iu (two real sequences per complex transform),
 //    the products q u, q v (src/produ.f90), and one real-to-complex transform
-//    of q u + i q v.
+//    of q u + i q v.  At N = 8192 a workgroup takes one row, as the 4096-point
+//    complex transform of its even and odd samples (r_pass_row).
 //
 // Between the passes the three (resp. two) fields live in HBM in blocks of
 // kYR rows x BT columns, [field][y / kYR][tile][y % kYR][kx % BT]: the y
@@ -639,7 +640,129 @@ __global__ void __launch_bounds__(rows_of(LOGN) * (1 << LOGN) / kE)
                                        tw);
 }
 
-// One row pair per workgroup: two workgroups share a CU.  Twiddles come
+// N = 8192, one row per workgroup.  A real row x of N points is the
+// M = N/2-point complex sequence z[m] = x[2m] + i x[2m+1]; with
+// W = exp(-2 pi i / N) its transform is
+//   Z[k] = E[k] + i O[k],  E[k] = X[k] + conj(X[M-k]),
+//                          O[k] = (X[k] - conj(X[M-k])) W^-k,
+// so the C2R transform of a row is one 4096-point complex transform, and
+// the R2C transform of a product p the same backwards: from
+// ZP = FFT(p_even + i p_odd),
+//   P[k] = PE[k] + W^k PO[k],  PE[k] = (ZP[k] + conj(ZP[M-k])) / 2,
+//                              PO[k] = (ZP[k] - conj(ZP[M-k])) / 2i.
+// Products stay pointwise in z (even samples in .x, odd in .y), as for row
+// pairs.  A row needs one 4096-point line of LDS and 512 threads: two
+// workgroups share a CU, where a row pair of 8192-point lines fills one.
+// The rectangle lies below 3N/8 (X[k] = 0 from k = 3M/4 on).
+
+// W^(j + 512 r) from wj = W^j (8192-point): W^(512 r) = exp(-i pi r / 8).
+template <int R>
+__device__ __forceinline__ double2 w_step(double2 wj) {
+  constexpr double C1 = 0.92387953251128675613, S1 = 0.38268343236508977173;
+  constexpr double H = 0.70710678118654752440;
+  if constexpr (R == 0) return wj;
+  if constexpr (R == 4) return make_double2(wj.y, -wj.x);
+  constexpr double c = R == 1 ? C1 : R == 2 ? H : R == 3 ? S1 : R == 5 ? -S1 : R == 6 ? -H : -C1;
+  constexpr double s = R == 1 ? -S1 : R == 2 ? -H : R == 3 ? -C1 : R == 5 ? -C1 : R == 6 ? -H : -S1;
+  return cmul(wj, make_double2(c, s));
+}
+// z[R..7] = E + i O from A = X[k], B = X[M-k], k = j + 512 R.
+template <int R>
+__device__ __forceinline__ void row_pack(double2* z, const double2* A, const double2* B,
+                                         double2 wj) {
+  if constexpr (R < kE) {
+    const double2 a = A[R], b = B[R];
+    const double2 e = make_double2(a.x + b.x, a.y - b.y);
+    const double2 o = cmulc(make_double2(a.x - b.x, a.y + b.y), w_step<R>(wj));
+    z[R] = make_double2(e.x - o.y, e.y + o.x);
+    row_pack<R + 1>(z, A, B, wj);
+  }
+}
+// Z of one row of a field (element k at off + kYR k: tiles of one column).
+__device__ __forceinline__ void row_build(double2* z, int j, const double2* __restrict__ y,
+                                          unsigned off, int kxm,
+                                          const double2* __restrict__ tw) {
+  constexpr int M = 4096;
+  asm volatile("" : "+v"(j));  // offsets formed here, not held from another field's
+  const double2 wj = tw[j];
+  double2 A[kE], B[kE];
+#pragma unroll
+  for (int r = 0; r < kE; ++r) {  // all loads in flight together
+    const int k = j + 512 * r, mk = M - k;
+    A[r] = B[r] = make_double2(0.0, 0.0);
+    if (r < 6 && k <= kxm) A[r] = y[off + ycol_off(k, 1)];    // k < 3M/4
+    if (r >= 2 && mk <= kxm) B[r] = y[off + ycol_off(mk, 1)];  // M - k < 3M/4
+  }
+  if (j == 0) A[0].y = 0.0;  // Im X[0] is not used (C2R)
+  row_pack<0>(z, A, B, wj);
+}
+// Store P[k], k <= kx_max, of the product whose ZP the thread holds in
+// natural order (k = j + 512 r).  ZP[M-k] comes through LDS.
+__device__ __forceinline__ void row_split_store(const double2* w, double2* buf, int j,
+                                                double2* __restrict__ y, unsigned off,
+                                                int kxm, const double2* __restrict__ tw) {
+  constexpr int TPL = 512;
+  asm volatile("" : "+v"(j));
+  const double2 wj = tw[j];
+  double2* pj = buf + lpad(j);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) pj[lpad_off(TPL * r)] = w[r];
+  __syncthreads();
+  const double2* pm = buf + lpad(TPL - j);
+  auto one = [&](auto RC) {
+    constexpr int r = decltype(RC)::value;
+    const int k = j + TPL * r;
+    if (k <= kxm) {
+      const double2 W = w[r];
+      const double2 Wc = mirror_at<12>(buf, pm, j, r);
+      const double2 pe = make_double2(0.5 * (W.x + Wc.x), 0.5 * (W.y - Wc.y));
+      const double2 po = make_double2(0.5 * (W.y + Wc.y), -0.5 * (W.x - Wc.x));
+      y[off + ycol_off(k, 1)] = cadd(pe, cmul(w_step<r>(wj), po));
+    }
+  };
+  one(std::integral_constant<int, 0>{});
+  one(std::integral_constant<int, 1>{});
+  one(std::integral_constant<int, 2>{});
+  one(std::integral_constant<int, 3>{});
+  one(std::integral_constant<int, 4>{});
+  one(std::integral_constant<int, 5>{});
+  __syncthreads();
+}
+__device__ __forceinline__ void row_body13(const PassArgs& a, int y, int j, double2* buf,
+                                           const double2* __restrict__ tw) {
+  constexpr int N = 8192;
+  const int kxm = a.rc.kx;
+  const unsigned fstride = (unsigned)a.ntiles * N;  // one field (tiles of one column)
+  const unsigned ro = yrow_off(y, a.ntiles, 1);
+  double2 z[kE], qq[kE];
+  row_build(z, j, a.y1, ro, kxm, tw);  // Q
+  fft_dif<+1, 2>(z, buf, j, tw, false);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) qq[r] = z[r];  // q_even + i q_odd
+  row_build(z, j, a.y1, 2 * fstride + ro, kxm, tw);  // V
+  fft_dif<+1, 2>(z, buf, j, tw, true);
+#pragma unroll
+  for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
+  fft_dit<-1, false, 2>(z, buf, j, tw);
+  __syncthreads();  // every wave's reads done before the split's writes
+  row_split_store(z, buf, j, a.y2, fstride + ro, kxm, tw);  // q v
+  row_build(z, j, a.y1, fstride + ro, kxm, tw);  // U
+  fft_dif<+1, 2>(z, buf, j, tw, false);  // the split ends with a barrier
+#pragma unroll
+  for (int r = 0; r < kE; ++r) z[r] = make_double2(qq[r].x * z[r].x, qq[r].y * z[r].y);
+  fft_dit<-1, false, 2>(z, buf, j, tw);
+  __syncthreads();
+  row_split_store(z, buf, j, a.y2, ro, kxm, tw);  // q u
+}
+template <int LOGN>
+__global__ void __launch_bounds__(512, 4) r_pass_row(PassArgs a) {
+  static_assert(LOGN == 13, "one-row x pass: 8192-point rows only");
+  __shared__ double2 lds[kIpLine];
+  row_body13(a, xcd_remap(blockIdx.x, gridDim.x), threadIdx.x, lds, a.tw);
+}
+
+// One row pair per workgroup: two workgroups share a CU up to N = 4096 (at
+// 8192 one fills it: the rectangle wider than 3N/8 only).  Twiddles come
 // from memory, each transform pass loading the next pass's entry ahead.
 template <int LOGN, int TR>
 __global__ void __launch_bounds__((1 << LOGN) / kE, 4)
@@ -1071,12 +1194,16 @@ hipError_t c_pass_launch(int mode, const PassArgs& a, hipStream_t s) {
 
 template <int L>
 hipError_t r_pass_launch(const PassArgs& a, hipStream_t s) {
-  if constexpr (L >= 11) {  // one row pair per workgroup, two workgroups per CU
+  if constexpr (L >= 11) {  // one row pair per workgroup (8192: one row)
     constexpr int T = (1 << L) / kE;
-    if (rect_tr(a.rc, 1 << L))
-      r_pass_pair<L, 1><<<dim3((1 << L) / 2), dim3(T), 0, s>>>(a);
-    else
+    if (rect_tr(a.rc, 1 << L)) {
+      if constexpr (L == 13)
+        r_pass_row<L><<<dim3(1 << L), dim3(T / 2), 0, s>>>(a);
+      else
+        r_pass_pair<L, 1><<<dim3((1 << L) / 2), dim3(T), 0, s>>>(a);
+    } else {
       r_pass_pair<L, 0><<<dim3((1 << L) / 2), dim3(T), 0, s>>>(a);
+    }
     return hipGetLastError();
   }
   constexpr int RP = rows_of(L), T = RP * (1 << L) / kE;
