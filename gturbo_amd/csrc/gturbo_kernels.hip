@@ -71,19 +71,20 @@ __device__ __forceinline__ constexpr bool live_r(int r) {
   return TR == 0 || (r != 3 && r != 4);
 }
 
-// Cache use of the y passes at N = 4096, where a pass moves more than the
-// 256 MB Infinity Cache holds.  The state z, the RK4 accumulator znew and
-// the integrating factor emk are read and written once per y pass and next
-// used a whole x pass later, by when they have left the caches; the
-// products Y2 are read once.  The y passes access all of them without
+// Cache use of the y passes at N = 4096 and 8192, where a pass moves more
+// than the 256 MB Infinity Cache holds.  The state z, the RK4 accumulator
+// znew and the integrating factor emk are read and written once per y pass
+// and next used a whole x pass later, by when they have left the caches;
+// the products Y2 are read once.  The y passes access them without
 // allocating cache lines (non-temporal), which leaves the caches to the
-// fields Y1 they write and the next x pass reads at once.  Measured at
-// 4096^2 against the same kernels with ordinary accesses, one process on
-// one box, results bit-identical: the state accesses alone y stages -8%,
-// RK4 step -2.4%; the Y2 reads on top of that x pass -4%, step -1.3%;
-// both together RK4 -3.5%, RK2 -2%.  (At N = 8192, whose passes use other
-// layouts, the same made the step 4% slower; non-temporal loads in the x
-// pass slow it by 9-22%.)
+// fields Y1 they write and the next x pass reads at once.  Measured against
+// the same kernels with ordinary accesses, one process on one box, results
+// bit-identical: at 4096^2 the state accesses alone y stages -8%, RK4 step
+// -2.4%; the Y2 reads on top of that x pass -4%, step -1.3%; both together
+// RK4 -3.5%, RK2 -2%.  At 8192^2 the state accesses RK4 step -1.7%, but
+// the Y2 reads on top +2.8% (there a one-column y workgroup reads half of
+// each Y2 line and the next column's workgroup the other half), so there
+// only the state bypasses the caches.  Non-temporal loads in the x pass slow it by 9-22%.
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 template <bool NT>
 __device__ __forceinline__ double2 ld_state(const double2* p) {
@@ -201,7 +202,8 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
   constexpr bool INV = MODE == C_START || MODE == C_RK4_1 || MODE == C_RK4_2 ||
                        MODE == C_RK4_3 || MODE == C_RK2_1 || MODE == C_RK4_4S ||
                        MODE == C_RK2_2S;
-  constexpr bool NT = LOGN == 12;  // state and Y2 accesses that bypass the caches
+  constexpr bool NT = LOGN >= 12;    // state accesses that bypass the caches
+  constexpr bool NT2 = LOGN == 12;   // Y2 reads likewise
   int b, j;
   if constexpr (IP && BT == 2) {
     b = (tid >> 6) & 1;
@@ -231,7 +233,7 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
     const unsigned colo = ycol_off(c, BT);
     const double2* srcA = a.y2;
 #pragma unroll
-    for (int r = 0; r < kE; ++r) z[r] = ld_state<NT>(&srcA[yrow_off(j + TPL * r, a.ntiles, BT) + colo]);
+    for (int r = 0; r < kE; ++r) z[r] = ld_state<NT2>(&srcA[yrow_off(j + TPL * r, a.ntiles, BT) + colo]);
     if constexpr (IP && LOGN == 12) fft_dit<-1, false>(z, buf, j, ytw);
     else if constexpr (IP) fft_dit2<-1, false>(z, buf, j, tw);
     else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
@@ -239,7 +241,7 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
     for (int r = 0; r < kE; ++r) A[r] = live_r<TR>(r) ? z[r] : make_double2(0.0, 0.0);
     const double2* srcB = a.y2 + (size_t)a.ntiles * N * BT;
 #pragma unroll
-    for (int r = 0; r < kE; ++r) z[r] = ld_state<NT>(&srcB[yrow_off(j + TPL * r, a.ntiles, BT) + colo]);
+    for (int r = 0; r < kE; ++r) z[r] = ld_state<NT2>(&srcB[yrow_off(j + TPL * r, a.ntiles, BT) + colo]);
     if constexpr (IP && LOGN == 12) fft_dit<-1, true>(z, buf, j, ytw);
     else if constexpr (IP) fft_dit2<-1, true>(z, buf, j, tw);
     else fft_reg<LOGN, -1, SCHED>(z, buf, j, tw);
