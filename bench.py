@@ -374,6 +374,11 @@ def adjoint_bench():
             "finite": bool(np.all(np.isfinite(g)))}
 
 
+def progress(msg):
+    """One line per finished leg, on stderr (stdout carries the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -546,20 +551,27 @@ def main():
         "kernels_sample_steps": args.breakdown_steps,
     }
     s.close()
+    progress(f"RK{p.rko} at {p.n}^2: {out['ms_per_step']:.3f} ms per step")
     if args.rk2_steps > 0 and args.rko == 4:
         out["rk2"] = workload_rate("euler4096_rk2", args.rk2_steps, 5, n=args.n)
+        progress("RK2 done")
     if args.config_steps:
         out["configs"] = {"euler256": workload_rate("euler256", 1000, 10),
                           "sqg1024": workload_rate("sqg1024", 2000, 10)}
+        progress("256^2 and 1024^2 workloads done")
     if args.ens_steps > 0:
         out["ensemble"] = ensemble_bench(args.ens_steps)
+        progress("ensemble done")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench()
+        progress("adjoint done")
     if args.config_steps and args.n == 4096:
         out["diagnostics"] = diag_bench()
+        progress("diagnostics done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, w0, w.forcing_seed)
         out["cpu_baseline"]["gpu_over_cpu"] = rate / out["cpu_baseline"]["value"]
+        progress("CPU baseline done")
     print(json.dumps(out))
 
 
