@@ -40,6 +40,8 @@ struct FEntry {
   double ff1, ff2;
 };
 
+struct EnsForce;
+
 // Arguments of the column (y) and row (x) passes of the nonlinear term.
 struct PassArgs {
   Rect rc;
@@ -66,6 +68,11 @@ struct PassArgs {
   const FEntry* fce;
   const double2* fph;
   double famp;
+  // batched ensembles: the phases and amplitude come from the member's
+  // forcing descriptor instead (fens->phases[*fstep * fens->nf ..],
+  // fens->amp), at the step counter the graph advances on the device
+  const EnsForce* fens;
+  const int* fstep;
 };
 
 // Arguments of the adjoint passes (transposed nonlinear term inside the

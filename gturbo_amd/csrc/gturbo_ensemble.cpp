@@ -4,10 +4,11 @@
 // Every member keeps its own compact fields (z, znew), y-pass buffers and
 // tables; a device array of per-member pass arguments and two work lists
 // (y tiles, x row groups, largest N first) let one launch of a batched pass
-// kernel cover every member.  The RK step of the whole ensemble -- five y
-// passes and four x passes for RK4, then the forcing of every member and
-// the step counter -- is captured once in a HIP graph (a few steps per
-// graph) and replayed.  The forcing phases are drawn on the host, member by
+// kernel cover every member.  The RK steps of the whole ensemble -- five y
+// passes and four x passes each for RK4, then the forcing of every member
+// and the step counter, with a step's last y pass, its forcing and the next
+// step's first y pass fused into one inside a run of steps -- are captured
+// once in a HIP graph (a few steps per graph) and replayed.  The forcing phases are drawn on the host, member by
 // member in the reference's order, for chunks of steps; a chunk is drawn
 // while the GPU runs the previous one (two buffers).
 #include <hip/hip_runtime.h>
@@ -116,6 +117,12 @@ int apply(gt_ens* e, const int32_t* cfg) {
     a.out = nullptr;
     a.ka = mb.ka;
     a.xvar = (cfg[4] >> mb.logn) & 1;
+    // the fused step boundary forces the member's columns with the phases
+    // of the step the device counter is at
+    a.fcs = mb.tab->fcs;
+    a.fce = mb.tab->fce;
+    a.fens = e->fdesc + m;
+    a.fstep = e->step;
     d[m] = a;
   }
   for (int m : order) {
@@ -152,9 +159,12 @@ int grid_of(const gt_ens* e, int nitems, int threads) {
   return std::max(1, std::min(nitems, g));
 }
 
-// One RK step of every member (and, with forcing, its forcing and the step
-// counter) on the ensemble stream.
-int launch_step(gt_ens* e, bool forcing) {
+// n RK steps of every member on the ensemble stream; with forcing, each
+// step's forcing and the step counter follow it.  Between two forced steps
+// the last stage, the forcing and the next step's first pass run as one y
+// pass (C_RK4_4S / C_RK2_2S), as gt_step4 / gt_step fuse a driver's step,
+// forcing, step: one launch fewer and the state read and written once.
+int launch_steps(gt_ens* e, int n, bool forcing) {
   const int ty = e->cfg[0], tx = e->cfg[1], sc = e->cfg[3];
   const uint32_t* yi = e->items;
   const uint32_t* xi = e->items + e->ny;
@@ -167,24 +177,28 @@ int launch_step(gt_ens* e, bool forcing) {
     HIPCHK(gt::launch_ens_r_pass(tx, ty, sc, e->descs, xi, e->nx, gx, e->stream));
     return GT_OK;
   };
-  CHK(cp(gt::C_START));
-  CHK(rp());
-  if (e->rko == 4) {  // src/rkstep.f90:82-131
-    CHK(cp(gt::C_RK4_1));
+  for (int s = 0; s < n; ++s) {
+    const bool fuse = forcing && s + 1 < n;  // this step's end with the next's start
+    if (s == 0) CHK(cp(gt::C_START));
     CHK(rp());
-    CHK(cp(gt::C_RK4_2));
-    CHK(rp());
-    CHK(cp(gt::C_RK4_3));
-    CHK(rp());
-    CHK(cp(gt::C_RK4_4));
-  } else {  // src/rkstep.f90:2-76
-    CHK(cp(gt::C_RK2_1));
-    CHK(rp());
-    CHK(cp(gt::C_RK2_2));
-  }
-  if (forcing) {  // src/main.f90:136
-    HIPCHK(gt::launch_ens_forcing(e->fdesc, e->nmem, e->ftotal, e->step, e->stream));
-    HIPCHK(gt::launch_ens_tick(e->step, e->stream));
+    if (e->rko == 4) {  // src/rkstep.f90:82-131
+      CHK(cp(gt::C_RK4_1));
+      CHK(rp());
+      CHK(cp(gt::C_RK4_2));
+      CHK(rp());
+      CHK(cp(gt::C_RK4_3));
+      CHK(rp());
+      CHK(cp(fuse ? gt::C_RK4_4S : gt::C_RK4_4));
+    } else {  // src/rkstep.f90:2-76
+      CHK(cp(gt::C_RK2_1));
+      CHK(rp());
+      CHK(cp(fuse ? gt::C_RK2_2S : gt::C_RK2_2));
+    }
+    if (forcing) {  // src/main.f90:136
+      if (!fuse)
+        HIPCHK(gt::launch_ens_forcing(e->fdesc, e->nmem, e->ftotal, e->step, e->stream));
+      HIPCHK(gt::launch_ens_tick(e->step, e->stream));
+    }
   }
   return GT_OK;
 }
@@ -193,7 +207,7 @@ int capture(gt_ens* e) {
   drop_graph(e);
   HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
   int rc = GT_OK;
-  for (int s = 0; s < e->cfg[5] && rc == GT_OK; ++s) rc = launch_step(e, true);
+  rc = launch_steps(e, e->cfg[5], true);
   hipGraph_t g = nullptr;
   const hipError_t ee = hipStreamEndCapture(e->stream, &g);
   if (rc != GT_OK) {
@@ -263,9 +277,9 @@ int time_steps(gt_ens* e, int steps, const std::vector<double2*>& backup, double
       HIPCHK(hipMemcpyAsync(mb.z, backup[m], sizeof(double2) * mb.rc.kc() * mb.kyp,
                             hipMemcpyDeviceToDevice, e->stream));
     }
-    CHK(launch_step(e, false));
+    CHK(launch_steps(e, 1, false));
     HIPCHK(hipEventRecord(e->t0, e->stream));
-    for (int s = 0; s < steps; ++s) CHK(launch_step(e, false));
+    CHK(launch_steps(e, steps, false));
     HIPCHK(hipEventRecord(e->t1, e->stream));
     HIPCHK(hipEventSynchronize(e->t1));
     float t = 0.f;
@@ -503,9 +517,9 @@ int gt_ens_advance(gt_ens* e, int32_t nsteps, int32_t hold) {
     HIPCHK(hipEventRecord(e->ph_ev[b], e->stream));
     e->ph_used[b] = true;
     for (int i = 0; i < full / gs; ++i) HIPCHK(hipGraphLaunch(e->gexec, e->stream));
-    for (int i = 0; i < full % gs; ++i) CHK(launch_step(e, true));
+    CHK(launch_steps(e, full % gs, true));
     if (full < ch) {
-      CHK(launch_step(e, false));
+      CHK(launch_steps(e, 1, false));
       e->pending = true;
     }
     left -= ch;

@@ -168,11 +168,17 @@ __device__ __forceinline__ void force_column(const PassArgs& a, int c, bool colo
 #pragma clang fp contract(off)
   constexpr int N = 1 << LOGN, TPL = N / kE;
   if (a.fcs == nullptr || !colok) return;
+  const double2* ph = a.fph;
+  double amp = a.famp;
+  if (a.fens != nullptr) {  // a batched ensemble member: this step's phases
+    ph = a.fens->phases + (size_t)(*a.fstep) * a.fens->nf;
+    amp = a.fens->amp;
+  }
   const int e1 = a.fcs[c + 1];
   for (int e = a.fcs[c]; e < e1; ++e) {
     const FEntry f = a.fce[e];
-    const double2 cs = a.fph[f.k];
-    const double a1 = a.famp * f.ff1 * cs.x, a2 = a.famp * f.ff2 * cs.y;
+    const double2 cs = ph[f.k];
+    const double a1 = amp * f.ff1 * cs.x, a2 = amp * f.ff2 * cs.y;
 #pragma unroll
     for (int r = 0; r < kE; ++r) {
       if (live_r<TR>(r) && row_of(j + TPL * r, a.rc, N) == f.row) {
@@ -1383,6 +1389,8 @@ hipError_t ens_c_launch(int mode, const PassArgs* d, const uint32_t* it, int n, 
     case C_RK4_4: ens_c_pass<T, C_RK4_4, SCHED><<<g, b, 0, s>>>(d, it, n); break;
     case C_RK2_1: ens_c_pass<T, C_RK2_1, SCHED><<<g, b, 0, s>>>(d, it, n); break;
     case C_RK2_2: ens_c_pass<T, C_RK2_2, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK4_4S: ens_c_pass<T, C_RK4_4S, SCHED><<<g, b, 0, s>>>(d, it, n); break;
+    case C_RK2_2S: ens_c_pass<T, C_RK2_2S, SCHED><<<g, b, 0, s>>>(d, it, n); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
