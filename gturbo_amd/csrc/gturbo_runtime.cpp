@@ -72,7 +72,7 @@ hipEvent_t get_event(gt_ctx* c) {
 Timer::Timer(gt_ctx* c_, int cls_) : c(c_), cls(cls_) {
   if (c->timing && (c->timing_mask >> cls & 1u)) {
     a = get_event(c);
-    if (a) hipEventRecord(a, c->stream);
+    if (a) (void)hipEventRecord(a, c->stream);
   }
 }
 
@@ -80,7 +80,7 @@ Timer::~Timer() {
   if (a) {
     hipEvent_t b = get_event(c);
     if (b) {
-      hipEventRecord(b, c->stream);
+      (void)hipEventRecord(b, c->stream);
       c->tl.push_back({cls, a, b});
     }
   }
@@ -240,8 +240,8 @@ int ensure_ybuf(gt_ctx* c, const Rect& r) {
   const size_t cols = std::max(ntiles * bt, (size_t)gt::y1_cols(r.kc()));
   const size_t need = cols * (size_t)c->n;
   if (need <= c->ycap) return GT_OK;
-  if (c->y1) hipFree(c->y1);
-  if (c->y2) hipFree(c->y2);
+  if (c->y1) (void)hipFree(c->y1);
+  if (c->y2) (void)hipFree(c->y2);
   c->y1 = c->y2 = nullptr;
   c->ycap = 0;
   HIPCHK(hipMalloc(&c->y1, sizeof(double2) * need * 3));
@@ -266,11 +266,11 @@ int lookup(gt_ctx* c, const void* host, Mirror** m) {
   return GT_OK;
 }
 
-int alloc_cmp(gt_ctx* c, Mirror& m, const Rect& r) {
+int alloc_cmp(gt_ctx*, Mirror& m, const Rect& r) {
   const int kyp = kyp_of(r);
   const size_t need = (size_t)r.kc() * kyp;
   if (need > m.cmp_cap) {
-    if (m.dcmp) hipFree(m.dcmp);
+    if (m.dcmp) (void)hipFree(m.dcmp);
     m.dcmp = nullptr;
     m.cmp_cap = 0;
     HIPCHK(hipMalloc(&m.dcmp, sizeof(double2) * need));
@@ -378,7 +378,7 @@ int ensure_diag(gt_ctx* c, const Rect& r) {
   const size_t nbin = c->n / 2;
   const size_t need = 2 * (size_t)(r.kc() + 1) * nbin + 2 * nbin + (size_t)r.kc() * 6 + 6;
   if (need <= c->dg_cap) return GT_OK;
-  if (c->dg) hipFree(c->dg);
+  if (c->dg) (void)hipFree(c->dg);
   c->dg = nullptr;
   c->dg_cap = 0;
   HIPCHK(hipMalloc(&c->dg, sizeof(double) * need));
@@ -620,34 +620,34 @@ int gt_create(int32_t n, const double* p, gt_ctx** out) {
 int gt_destroy(gt_ctx* c) {
   if (!c) return GT_OK;
   c->pend = Pending{};  // a deferred step end is dropped with the context
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   adj_free(c);
   for (auto& kv : c->present) {
     Mirror& m = kv.second;
-    if (m.owned && m.dref) hipFree(m.dref);
-    if (m.dcmp) hipFree(m.dcmp);
+    if (m.owned && m.dref) (void)hipFree(m.dref);
+    if (m.dcmp) (void)hipFree(m.dcmp);
   }
   for (auto& t : c->tab) {
-    if (t.emk) hipFree(t.emk);
-    if (t.modes) hipFree(t.modes);
-    if (t.fcs) hipFree(t.fcs);
-    if (t.fce) hipFree(t.fce);
+    if (t.emk) (void)hipFree(t.emk);
+    if (t.modes) (void)hipFree(t.modes);
+    if (t.fcs) (void)hipFree(t.fcs);
+    if (t.fce) (void)hipFree(t.fce);
   }
-  if (c->tw) hipFree(c->tw);
-  if (c->y1) hipFree(c->y1);
-  if (c->y2) hipFree(c->y2);
-  if (c->flag) hipFree(c->flag);
-  if (c->dg) hipFree(c->dg);
-  if (c->ph_host) hipHostFree(c->ph_host);
-  if (c->ph_dev) hipFree(c->ph_dev);
+  if (c->tw) (void)hipFree(c->tw);
+  if (c->y1) (void)hipFree(c->y1);
+  if (c->y2) (void)hipFree(c->y2);
+  if (c->flag) (void)hipFree(c->flag);
+  if (c->dg) (void)hipFree(c->dg);
+  if (c->ph_host) (void)hipHostFree(c->ph_host);
+  if (c->ph_dev) (void)hipFree(c->ph_dev);
   for (int i = 0; i < kRing; ++i)
-    if (c->ph_ev[i]) hipEventDestroy(c->ph_ev[i]);
+    if (c->ph_ev[i]) (void)hipEventDestroy(c->ph_ev[i]);
   for (auto& t : c->tl) {
-    hipEventDestroy(t.a);
-    hipEventDestroy(t.b);
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
   }
-  for (auto e : c->ev_pool) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return GT_OK;
 }
@@ -1123,7 +1123,7 @@ int gt_shell_index(gt_ctx* c, int32_t* ib) {
   if (e == hipSuccess)
     e = hipMemcpyAsync(ib, d, sizeof(int) * tot, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  hipFree(d);
+  (void)hipFree(d);
   if (e != hipSuccess) return fail(GT_ERR_HIP, "shell_index: %s", hipGetErrorString(e));
   return GT_OK;
 }
