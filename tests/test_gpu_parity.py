@@ -375,20 +375,25 @@ def test_4096_nlt_and_steps_on_each_transform_path(oracle_mod, alpv, truncate):
         s.close()
 
 
-@pytest.mark.parametrize("alpv", [2.0, 1.0])
-def test_8192_nlt_and_steps(oracle_mod, alpv):
+@pytest.mark.parametrize("alpv,truncate", [(2.0, 1), (1.0, 1), (2.0, 0)])
+def test_8192_nlt_and_steps(oracle_mod, alpv, truncate):
     """N = 8192, the reference's compile-time default (src/modules.f90): nlt,
-    one RK4 step and one RK2 step with forcing against the oracle."""
+    one RK4 step and one RK2 step with forcing against the oracle.  With the
+    2/3 rule the x pass transforms one row per workgroup (the 4096-point
+    transform of its even and odd samples) and the y passes run the in-place
+    transforms; without it (the whole half-spectrum kept) row pairs and the
+    Stockham transforms."""
     n = 8192
     for rko in (4, 2):
-        p = params_for(n, alpv=alpv, rko=rko, ring=(63.0, 65.0))  # nf within NFORC
+        p = params_for(n, alpv=alpv, rko=rko, truncate=truncate,
+                       ring=(63.0, 65.0))  # nf within NFORC
         s, o = _pair(oracle_mod, p)
-        z = random_spectral(n, 41 + rko)
+        z = random_spectral(n, 41 + rko, truncated=bool(truncate))
         zg = z.copy()
         u, v = s.field(), s.field()
         scratch = [s.field() for _ in range(4)]
         _present(s, zg, u, v, *scratch)
-        print("8192 alpha", alpv, "rko", rko, flush=True)
+        print("8192 alpha", alpv, "truncate", truncate, "rko", rko, flush=True)
         if rko == 4:
             zn, zng = z.copy(), z.copy()
             _present(s, zng)
