@@ -8,9 +8,10 @@
 // passes and four x passes each for RK4, then the forcing of every member
 // and the step counter, with a step's last y pass, its forcing and the next
 // step's first y pass fused into one inside a run of steps -- are captured
-// once in a HIP graph (a few steps per graph) and replayed.  The forcing phases are drawn on the host, member by
-// member in the reference's order, for chunks of steps; a chunk is drawn
-// while the GPU runs the previous one (two buffers).
+// once in a HIP graph (a few steps per graph) and replayed.  The forcing
+// phases are drawn on the host, member by member in the reference's order,
+// for chunks of steps; a chunk is drawn while the GPU runs the previous one
+// (two buffers).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -177,9 +178,10 @@ int launch_steps(gt_ens* e, int n, bool forcing) {
     HIPCHK(gt::launch_ens_r_pass(tx, ty, sc, e->descs, xi, e->nx, gx, e->stream));
     return GT_OK;
   };
+  bool started = false;  // this step's first pass ran fused with the last one's end
   for (int s = 0; s < n; ++s) {
     const bool fuse = forcing && s + 1 < n;  // this step's end with the next's start
-    if (s == 0) CHK(cp(gt::C_START));
+    if (!started) CHK(cp(gt::C_START));
     CHK(rp());
     if (e->rko == 4) {  // src/rkstep.f90:82-131
       CHK(cp(gt::C_RK4_1));
@@ -199,6 +201,7 @@ int launch_steps(gt_ens* e, int n, bool forcing) {
         HIPCHK(gt::launch_ens_forcing(e->fdesc, e->nmem, e->ftotal, e->step, e->stream));
       HIPCHK(gt::launch_ens_tick(e->step, e->stream));
     }
+    started = fuse;
   }
   return GT_OK;
 }
