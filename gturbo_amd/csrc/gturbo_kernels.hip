@@ -84,7 +84,8 @@ __device__ __forceinline__ constexpr bool live_r(int r) {
 // RK4 -3.5%, RK2 -2%.  At 8192^2 the state accesses RK4 step -1.7%, but
 // the Y2 reads on top +2.8% (there a one-column y workgroup reads half of
 // each Y2 line and the next column's workgroup the other half), so there
-// only the state bypasses the caches.  Non-temporal loads in the x pass slow it by 9-22%.
+// only the state bypasses the caches.  Non-temporal loads in the x pass
+// slow it by 9-22%.
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 template <bool NT>
 __device__ __forceinline__ double2 ld_state(const double2* p) {
@@ -210,7 +211,7 @@ __device__ __forceinline__ void c_body(const PassArgs& a, int tile, int tid,
                        MODE == C_RK4_3 || MODE == C_RK2_1 || MODE == C_RK4_4S ||
                        MODE == C_RK2_2S;
   constexpr bool NT = LOGN >= 12;    // state accesses that bypass the caches
-  constexpr bool NT2 = LOGN == 12;   // Y2 reads likewise
+  constexpr bool NT2 = LOGN == 12;   // Y2 reads likewise, at 4096 only
   int b, j;
   if constexpr (IP && BT == 2) {
     b = (tid >> 6) & 1;
